@@ -1,0 +1,123 @@
+"""ctypes binding of libcochlea_amd.so (the C-ABI declared in include/cochlea_amd.h).
+
+The library is built in-tree by ``cochlearmodel_amd.build``.  Loading fails
+loudly when it is missing: there is no CPU fallback for any product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcochlea_amd.so")
+
+SECTIONS = 256
+LAMBDA_COUNT = 3
+
+
+class BMScalars(C.Structure):
+    """cm_bm_scalars (cm_types.h)."""
+    _fields_ = [(n, C.c_float) for n in (
+        "w_ohc", "time_step", "time_step_out", "delta_x", "alpha_r", "Ts", "alpha_l", "alpha_s",
+        "Gme", "a0", "a1", "a2", "sigma_ow", "eta1", "eta2", "dt_min", "dt_max")] + [
+        ("j_fast", C.c_int32), ("j_slow", C.c_int32), ("max_passes", C.c_int32),
+        ("decision_mode", C.c_int32)]
+
+
+ITEM_DTYPE = np.dtype([("start", np.int32), ("mosi", np.int32), ("mesi", np.int32),
+                       ("write_start", np.int32), ("tol", np.float32), ("m1tol", np.float32),
+                       ("set", np.int32), ("pad", np.int32)])
+STATS_DTYPE = np.dtype([("steps", np.int64), ("passes", np.int64), ("sweeps", np.int64)])
+PROFILE_DTYPE = np.dtype([("dA", np.float64), ("recipdA", np.float64), ("dBSPLSignal", np.float64),
+                          ("dBSPLNoise", np.float64), ("Wn", np.float64), ("isReference", np.int8),
+                          ("pad0", np.int8, 3), ("calculateIndexPosition", np.int32),
+                          ("referenceIndex", np.int32), ("pad1", np.int32), ("frequency", np.float64)])
+assert PROFILE_DTYPE.itemsize == 64
+
+_lib = None
+
+vp = C.c_void_p
+i32, i64, f32, f64, sz = C.c_int32, C.c_int64, C.c_float, C.c_double, C.c_size_t
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "cm_last_error": (C.c_char_p, []),
+    "cm_set_stream": (C.c_int, [vp]),
+    "cm_device_synchronize": (C.c_int, []),
+    "cm_last_kernel_ms": (C.c_float, []),
+    "cm_set_bm_launch": (C.c_int, [C.c_int, C.c_int]),
+    "BMOHCKernel_Init": (C.c_int, [vp] * 11 + [f32, f32, sz, sz, sz, C.c_int, C.c_int, C.c_int]),
+    "BMOHCNewKernel": (C.c_int, [vp, C.c_int, f32, f32, f32, f32, f32, C.c_int, C.c_int, C.c_int, f32, f32, f32,
+                                 f32, f32, f32, f32, f32, f32, f32, C.c_int, C.c_int, C.c_int, C.c_long, C.c_int,
+                                 f32, f32, C.c_int, f32, f32, C.c_int, C.c_int, f32, f32, f32, C.c_int, C.c_int,
+                                 C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "cm_bm_run": (C.c_int, [vp, vp, i64, vp, vp, vp, C.c_int, vp, i64] + [vp] * 9 + [C.c_int, C.c_int, C.c_int]),
+    "cm_get_last_blocks": (C.c_int, [vp, C.c_int]),
+    "cm_get_last_stats": (C.c_int, [vp, C.c_int]),
+    "BMOHCKernel_Copy_Results": (C.c_int, [vp, sz, sz]),
+    "ReverseKernel_Copy_Results": (C.c_int, [vp, sz]),
+    "extractConvergenceTimes": (C.c_int, [vp, sz]),
+    "getCudaFailedTimeNodes": (vp, []),
+    "getCudaFailedBlocks": (vp, []),
+    "getCudaConvergeSpeedBlocks": (vp, []),
+    "getCudaConvergedTimeNodes": (vp, []),
+    "getCudaConvergedBlocks": (vp, []),
+    "getCudaConvergedJacobyLoopsPerIteration": (vp, []),
+    "getCudaConvergedJacobyLoopsPerIterationBlocks": (vp, []),
+    "getCudaBMVelocity": (vp, []),
+    "getCudaInputSamples": (vp, []),
+    "GeneralKernel_Copy_Results_Bytes": (C.c_int, [vp, vp, sz, sz]),
+    "GeneralKernel_Copy_Results": (C.c_int, [vp, vp, sz]),
+    "GeneralKernel_Copy_Results_Double": (C.c_int, [vp, vp, sz]),
+    "setupToleranceProfile": (C.c_int, [vp, C.c_int, f32, f32, C.c_int, f32, f32, C.c_int, C.c_int, C.c_int]),
+    "BMOHCKernel_Free": (C.c_int, []),
+    "cm_model_vectors": (C.c_int, [vp]),
+    "cm_tri_matrix": (C.c_int, [vp, vp, vp]),
+    "cm_pack_section_params": (C.c_int, [vp]),
+    "cm_default_bm_scalars": (C.c_int, [f32, vp]),
+}
+
+
+class CochleaError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libcochlea_amd.so; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise CochleaError(f"{LIB_PATH} is missing: run `python -m cochlearmodel_amd.build` "
+                               "(there is no CPU fallback)")
+        L = C.CDLL(LIB_PATH, mode=C.RTLD_GLOBAL)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def declared_symbols():
+    return list(_SIGS)
+
+
+def check(rc: int, what: str = ""):
+    if rc != 0:
+        msg = lib().cm_last_error().decode(errors="replace")
+        raise CochleaError(f"{what} failed ({rc}): {msg}")
+
+
+def ptr(a):
+    """Address of a numpy array, a torch tensor or None."""
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data_as(C.c_void_p)
+    if hasattr(a, "data_ptr"):
+        return C.c_void_p(a.data_ptr())
+    if isinstance(a, int):
+        return C.c_void_p(a)
+    raise TypeError(type(a))

@@ -1,0 +1,62 @@
+// Plain-C types shared by the HIP kernels, the C-ABI layer and the C++ host.
+// They carry the argument meaning of the reference's BM launch
+// (include/cochlea.cuh:13574-13620, src/cochlea.cu:990-1184) in struct form.
+#ifndef CM_TYPES_H
+#define CM_TYPES_H
+
+#include <stdint.h>
+
+#define CM_SECTIONS 256
+#define CM_LAMBDA_COUNT 3
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+// Scalar constants of one BM launch.
+typedef struct {
+  float w_ohc;
+  float time_step;      // initial dt
+  float time_step_out;  // sample clock: dt0 * int(1/(Fs*dt_max))
+  float delta_x;
+  float alpha_r;
+  float Ts;
+  float alpha_l;
+  float alpha_s;
+  float Gme;
+  float a0, a1, a2;
+  float sigma_ow;
+  float eta1, eta2;
+  float dt_min, dt_max;
+  int32_t j_fast;        // JACOBBY_Loops_Fast (passes >= 2)
+  int32_t j_slow;        // JACOBBY_Loops_Slow (passes 0, 1)
+  int32_t max_passes;    // Cuda_Outern_Loops
+  int32_t decision_mode; // 0: decision from the full reduction; 1: zero-partial lane outcome
+} cm_bm_scalars;
+
+// One time block (or one whole signal in continuous mode).
+typedef struct {
+  int32_t start;        // first input sample integrated from zero state
+  int32_t mosi;         // first output row (out_sample_index)
+  int32_t mesi;         // end row (end_sample_index)
+  int32_t write_start;  // first row this block owns in per-row outputs
+  float tol;            // throw tolerance
+  float m1tol;          // max m1 sp tolerance
+  int32_t set;          // parameter set (gamma vector) index
+  int32_t pad;
+} cm_bm_item;
+
+// Per-section packed parameters: 8 x 256 floats in this order.
+enum { CM_SEC_MASS = 0, CM_SEC_RSM, CM_SEC_R, CM_SEC_S, CM_SEC_Q, CM_SEC_SOHC, CM_SEC_STM, CM_SEC_RTM, CM_SEC_COUNT };
+
+// Per-block statistics written by the BM kernel.
+typedef struct {
+  int64_t steps;   // committed steps
+  int64_t passes;  // predictor/corrector passes
+  int64_t sweeps;  // Jacobi sweeps
+} cm_item_stats;
+
+#ifdef __cplusplus
+}
+#endif
+#endif

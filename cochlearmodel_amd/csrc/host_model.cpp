@@ -1,0 +1,124 @@
+// Host-side physical model: CModel vectors, the tridiagonal of the pressure
+// equation and the scalar constants Run_Cuda hands to the BM launch.
+//   CModel::CModel            src/model.cpp:18-86 (macros include/const.h:216-255)
+//   CSolver::Init_Tri_Matrix  src/solver.cpp:155-176
+//   parameter packing         src/solver.cpp:357-376, 629-659
+// The reference's constants are float-suffixed macros; they are evaluated in
+// float and promoted, as the reference does, before any FP64 arithmetic.
+#include <cmath>
+#include <cstring>
+
+#include "../../include/cochlea_amd.h"
+
+namespace {
+constexpr int NS = CM_SECTIONS;
+constexpr float kPI = 3.141592653589793f;  // include/const.h:142
+constexpr float kCochleaLen = 3.5f, kArea = 0.5f, kRho = 1.0f, kBeta = 0.003f;
+constexpr float kM0 = 1.286e-6f, kM1 = 1.5f, kR0 = 0.25f, kR1 = -0.06f, kS0 = 1.282e4f, kS1 = -1.5f;
+constexpr float kSigmaOW = 0.5f, kGammaOW = 20e3f, kGme = 21.4f, kCow = 6e-3f, kRatioK = 1e-3f;
+constexpr float kAlphaS = 1e-6f, kAlphaL = 2e-6f, kAlphaR = 0.0f;
+
+double w_ohc() { return (double)(2 * kPI * 1000.0f); }                  // W_OHC
+double dx() { return (double)(kCochleaLen / NS); }
+double bc0() { return (double)((2.0f * kRho * kCow) / kSigmaOW); }        // BC_0
+double bc1() {                                                            // BC_1
+  const float wow2 = 2 * kPI * 1500 * 2 * kPI * 1500;
+  const float cme_gme = (2 * kPI * 1340) * (2 * kPI * 1340) * 0.059f / (0.49f * 1.4f) * kGme;
+  return (double)(kSigmaOW * wow2 + cme_gme);
+}
+double bc2() { return (double)(kSigmaOW * kGammaOW); }                     // BC_2
+double eta1() { return (double)((1.0f) / kAlphaL * 2 * kPI * 1000.0f); }  // ETA_1
+}  // namespace
+
+extern "C" {
+
+int cm_model_vectors(double *out) {
+  if (!out) return -3;
+  const double d = dx();
+  for (int i = 0; i < NS; ++i) {
+    const double M = (double)kM0 * std::exp((double)kM1 * d * i);
+    const double R = (double)kR0 * std::exp((double)kR1 * d * i);
+    const double S = (double)kS0 * std::exp((double)kS1 * d * i);
+    const double wcf2 = S / M;
+    const double Stm = R * wcf2 / w_ohc();
+    out[0 * NS + i] = M;
+    out[1 * NS + i] = R;
+    out[2 * NS + i] = S;
+    out[3 * NS + i] = R;                        // R_tm = R_bm
+    out[4 * NS + i] = Stm;
+    out[5 * NS + i] = (double)kRatioK * Stm;    // S_ohc
+    out[6 * NS + i] = 2 * (double)kRho * (double)kBeta / ((double)kArea * M);   // Q
+  }
+  return 0;
+}
+
+int cm_tri_matrix(double *lower, double *mid, double *upper) {
+  double v[7 * NS];
+  cm_model_vectors(v);
+  const double d = dx();
+  for (int i = 0; i < NS; ++i) {
+    if (mid) mid[i] = -1.0 * (2.0 + d * d * v[6 * NS + i]);
+    if (upper) upper[i] = 1.0;
+    if (lower) lower[i] = 1.0;
+  }
+  if (mid) {
+    mid[0] = -1.0 * (1.0 + d * bc0());
+    mid[NS - 1] = 1.0;
+  }
+  if (lower) {
+    lower[NS - 1] = 0.0;
+    lower[NS - 2] = 0.0;
+  }
+  return 0;
+}
+
+// 8 x 256 floats: mass, 1/m_i, R, S, Q, S_ohc, S_tm, R_tm (src/solver.cpp:357-376).
+int cm_pack_section_params(float *out) {
+  if (!out) return -3;
+  double v[7 * NS], m[NS];
+  cm_model_vectors(v);
+  cm_tri_matrix(nullptr, m, nullptr);
+  for (int i = 0; i < NS; ++i) {
+    out[CM_SEC_MASS * NS + i] = (float)v[0 * NS + i];
+    out[CM_SEC_RSM * NS + i] = 1.0f / (float)m[i];
+    out[CM_SEC_R * NS + i] = (float)v[1 * NS + i];
+    out[CM_SEC_S * NS + i] = (float)v[2 * NS + i];
+    out[CM_SEC_Q * NS + i] = (float)v[6 * NS + i];
+    out[CM_SEC_SOHC * NS + i] = (float)v[5 * NS + i];
+    out[CM_SEC_STM * NS + i] = (float)v[4 * NS + i];
+    out[CM_SEC_RTM * NS + i] = (float)v[3 * NS + i];
+  }
+  return 0;
+}
+
+// Scalars of the BM launch for sample rate Fs (src/solver.cpp:437-438, 629-659;
+// CParams::cudaIterationsPerKernel, include/params.h:378).
+int cm_default_bm_scalars(float Fs, cm_bm_scalars *o) {
+  if (!o || !(Fs > 0)) return -3;
+  std::memset(o, 0, sizeof(*o));
+  const float dt_max = 1e-6f, dt_min = 1e-18f, dt0 = 1e-6f;
+  o->w_ohc = (float)w_ohc();
+  o->time_step = dt0;
+  o->time_step_out = dt0 * (float)(int)(1.0f / (Fs * dt_max));
+  o->delta_x = (float)dx();
+  o->alpha_r = kAlphaR;
+  o->Ts = (float)(1.0 / (double)Fs);
+  o->alpha_l = kAlphaL;
+  o->alpha_s = kAlphaS;
+  o->Gme = kGme;
+  o->a0 = (float)bc0();
+  o->a1 = (float)bc1();
+  o->a2 = (float)bc2();
+  o->sigma_ow = kSigmaOW;
+  o->eta1 = (float)eta1();
+  o->eta2 = 0.0f;
+  o->dt_min = dt_min;
+  o->dt_max = dt_max;
+  o->j_fast = 2;
+  o->j_slow = 20;
+  o->max_passes = 10;
+  o->decision_mode = 0;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,120 @@
+/*
+ * CPU oracle for the cochlear hot path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * This library is a plain CPU restatement of the reference's algorithm
+ * (YonatanKoral/CochlearModel, kernel Run_Fast_BM_Calculation=17 and the
+ * IHC / lambda / JND stages).  It exists so that tests, smoke() and the
+ * bench's cpu_baseline leg can check the HIP product against it.  Nothing
+ * in the product links, loads or calls it.
+ *
+ * Every function names the reference lines it follows.  Arithmetic is
+ * IEEE single (or double for the FP64 rebuild) with no FMA contraction:
+ * the build uses -ffp-contract=off and every fused multiply-add is written
+ * as fmaf()/fma() explicitly.
+ */
+#ifndef COCHLEA_ORACLE_H
+#define COCHLEA_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_SECTIONS 256
+#define ORC_LAMBDA_COUNT 3
+
+/* Scalar model constants of one BM launch (include/cochlea.cuh:13574-13620
+ * argument list; values packed by src/solver.cpp:629-659). */
+typedef struct {
+  float w_ohc;
+  float time_step;      /* initial dt */
+  float time_step_out;  /* sample clock (Q9) */
+  float delta_x;
+  float alpha_r;
+  float Ts;
+  float alpha_l;
+  float alpha_s;
+  float Gme;
+  float a0, a1, a2;
+  float sigma_ow;
+  float eta1, eta2;
+  float dt_min, dt_max;
+  int32_t j_fast;       /* JACOBBY_Loops_Fast */
+  int32_t j_slow;       /* JACOBBY_Loops_Slow */
+  int32_t max_passes;   /* Cuda_Outern_Loops */
+  int32_t decision_mode;/* 0 = lane-0 full reduction, 1 = zero-lane race outcome */
+} orc_bm_scalars;
+
+/* One work item = one reference "time block" (src/cochlea.cu:1148-1176). */
+typedef struct {
+  int32_t start;        /* first input sample integrated (zero state) */
+  int32_t mosi;         /* gen_model_out_sample_index */
+  int32_t mesi;         /* gen_model_end_sample_index */
+  int32_t write_start;  /* first row this item owns in per-row outputs */
+  float tol;            /* gen_model_throw_tolerance */
+  float m1tol;          /* gen_model_max_m1_sp_tolerance */
+  int32_t set;          /* parameter-set index (selects gamma) */
+  int32_t pad;
+} orc_bm_item;
+
+/* Model vectors of CModel::CModel (src/model.cpp:72-86), FP64.
+ * out layout: 7 x 256 doubles: M, R, S, R_tm, S_tm, S_ohc, Q. */
+int orc_model_vectors(double *out7x256);
+/* Scalar constants as the solver passes them (src/solver.cpp:629-659). */
+int orc_default_scalars(float Fs, orc_bm_scalars *out);
+/* Mid-diagonal of CSolver::Init_Tri_Matrix (src/solver.cpp:155-176). */
+int orc_tri_mid_diag(double *m256);
+
+/* Reference time-block decomposition (src/cochlea.cu:1148-1176) plus the
+ * disjoint row ownership used for per-row outputs. */
+int orc_block_decomposition(int time_blocks, int bufP1, int overlap, int decouple_filter,
+                            int32_t *start, int32_t *mosi, int32_t *mesi, int32_t *write_start);
+
+/* Kernel-17 restatement.  sec_f32 = 8 x 256 floats (mass, rsm, R, S, Q,
+ * S_ohc, S_tm, R_tm); gamma = n_sets x 256.  Outputs may be NULL. */
+int orc_bm_run_f32(const orc_bm_scalars *sc, const float *sec_f32, const float *gamma,
+                   const orc_bm_item *items, int n_items, const float *input, int64_t input_len,
+                   float *speeds, int64_t rows,
+                   int32_t *failed_nodes, float *conv_time, float *conv_loops, float *conv_jacobi,
+                   int32_t *failed_blocks, float *conv_time_blocks, float *conv_loops_blocks,
+                   float *conv_jacobi_blocks, int64_t *item_steps, int64_t *item_passes,
+                   int64_t *item_sweeps, int n_threads);
+/* FP64 rebuild of the same method (state and constants promoted from FP32). */
+int orc_bm_run_f64(const orc_bm_scalars *sc, const float *sec_f32, const float *gamma,
+                   const orc_bm_item *items, int n_items, const float *input, int64_t input_len,
+                   double *speeds, int64_t rows, int64_t *item_steps, int64_t *item_passes,
+                   int64_t *item_sweeps, int n_threads);
+
+/* Jacobi sweeps of the kernel convention (Q7) on one right-hand side. */
+int orc_jacobi(const float *Y, const float *rsm, float *p, int sweeps);
+
+/* tanh used by both the oracle and the GPU kernel (defined algorithm). */
+float orc_tanhf(float x);
+float orc_expf(float x);
+
+/* IHC + lambda (src/cochlea.cu:2712-2790, 3200-3240, 3365-3408, 3431-3506).
+ * bm: rows x 256 (row-major, section fastest).  lambda: 3 x rows x 256.
+ * stages (optional, may be NULL): 5 x rows x 256 = AC, dS, DC, PRE_IHC, IHC. */
+int orc_ihc_lambda(const float *bm, int64_t rows, int interval_len, int final_regular_division,
+                   const double *ac_b, int n_ac, const double *dc_b, int n_dc,
+                   const double *ihc_damage, const float *A3x256, const float *spont3,
+                   float lambda_sat, float eta_ac, float eta_dc, float scale_bm,
+                   float *lambda, float *stages, int n_threads);
+
+/* JND Fisher reductions (src/cochlea.cu:3741-3944) and final dB
+ * (src/AudioGramCreator.cpp:454-508). */
+int orc_jnd(const float *lambda, int64_t rows, int n_intervals, int interval_nodes, int head_nodes,
+            int length_nodes, const float *dA, const int32_t *ref_index, const double *nIHC,
+            const float *W3, double Fs, double delta_alpha_factor, int calc_ai, int calc_rms,
+            float *mean_rate, float *F_RA, float *FisherAISum);
+
+/* Profile input synthesis (src/cochlea.cu:4070-4103). */
+int orc_generate_input(int n_profiles, const double *dA, const double *Wn, const double *freq,
+                       const float *WN, double wn_dc, double wn_norm, double sig_norm,
+                       int interval_len, int head, float Fs, float *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

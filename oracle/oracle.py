@@ -1,0 +1,204 @@
+"""ctypes front end of the CPU oracle -- TEST INFRASTRUCTURE ONLY.
+
+Imported by tests/, ``__graft_entry__.smoke()`` and the cpu_baseline leg of
+``bench.py``.  The product (``cochlearmodel_amd``) never imports it.  See
+``oracle.h`` for the reference lines each function restates.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+SECTIONS = 256
+LAMBDA_COUNT = 3
+
+
+class BMScalars(C.Structure):
+    _fields_ = [(n, C.c_float) for n in (
+        "w_ohc", "time_step", "time_step_out", "delta_x", "alpha_r", "Ts", "alpha_l", "alpha_s",
+        "Gme", "a0", "a1", "a2", "sigma_ow", "eta1", "eta2", "dt_min", "dt_max")] + [
+        ("j_fast", C.c_int32), ("j_slow", C.c_int32), ("max_passes", C.c_int32),
+        ("decision_mode", C.c_int32)]
+
+
+ITEM_DTYPE = np.dtype([("start", np.int32), ("mosi", np.int32), ("mesi", np.int32),
+                       ("write_start", np.int32), ("tol", np.float32), ("m1tol", np.float32),
+                       ("set", np.int32), ("pad", np.int32)])
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = C.CDLL(_LIB_PATH)
+        _lib.orc_tanhf.restype = C.c_float
+        _lib.orc_tanhf.argtypes = [C.c_float]
+        _lib.orc_expf.restype = C.c_float
+        _lib.orc_expf.argtypes = [C.c_float]
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def tanhf(x: float) -> float:
+    return lib().orc_tanhf(C.c_float(x))
+
+
+def expf(x: float) -> float:
+    return lib().orc_expf(C.c_float(x))
+
+
+def model_vectors() -> np.ndarray:
+    out = np.zeros((7, SECTIONS), np.float64)
+    lib().orc_model_vectors(_p(out))
+    return out
+
+
+def tri_mid_diag() -> np.ndarray:
+    out = np.zeros(SECTIONS, np.float64)
+    lib().orc_tri_mid_diag(_p(out))
+    return out
+
+
+def default_scalars(Fs: float = 44100.0) -> BMScalars:
+    s = BMScalars()
+    lib().orc_default_scalars(C.c_float(Fs), C.byref(s))
+    return s
+
+
+def section_params() -> np.ndarray:
+    """8 x 256 float32: mass, 1/m_i, R, S, Q, S_ohc, S_tm, R_tm (src/solver.cpp:357-376)."""
+    v = model_vectors()
+    m = tri_mid_diag()
+    out = np.zeros((8, SECTIONS), np.float32)
+    out[0] = v[0].astype(np.float32)
+    out[1] = np.float32(1.0) / m.astype(np.float32)
+    out[2] = v[1].astype(np.float32)
+    out[3] = v[2].astype(np.float32)
+    out[4] = v[6].astype(np.float32)
+    out[5] = v[5].astype(np.float32)
+    out[6] = v[4].astype(np.float32)
+    out[7] = v[3].astype(np.float32)
+    return out
+
+
+def block_decomposition(time_blocks: int, bufP1: int, overlap: int, decouple_filter: int):
+    a = [np.zeros(time_blocks, np.int32) for _ in range(4)]
+    rc = lib().orc_block_decomposition(time_blocks, bufP1, overlap, decouple_filter, *[_p(x) for x in a])
+    assert rc == 0
+    return a
+
+
+def make_items(start, mosi, mesi, write_start, tol, m1tol, sets=None) -> np.ndarray:
+    n = len(start)
+    it = np.zeros(n, ITEM_DTYPE)
+    it["start"], it["mosi"], it["mesi"], it["write_start"] = start, mosi, mesi, write_start
+    it["tol"], it["m1tol"] = tol, m1tol
+    it["set"] = 0 if sets is None else sets
+    return it
+
+
+def bm_run(sc: BMScalars, sec: np.ndarray, gamma: np.ndarray, items: np.ndarray, inp: np.ndarray,
+           rows: int, precision: str = "f32", threads: int = 0, diagnostics: bool = False):
+    """Run the kernel-17 restatement; returns a dict of outputs."""
+    sec = np.ascontiguousarray(sec, np.float32)
+    gamma = np.ascontiguousarray(np.atleast_2d(gamma), np.float32)
+    items = np.ascontiguousarray(items, ITEM_DTYPE)
+    inp = np.ascontiguousarray(inp, np.float32)
+    n = len(items)
+    steps = np.zeros(n, np.int64)
+    passes = np.zeros(n, np.int64)
+    sweeps = np.zeros(n, np.int64)
+    out = {"steps": steps, "passes": passes, "sweeps": sweeps}
+    if precision == "f32":
+        speeds = np.zeros((rows, SECTIONS), np.float32)
+        fn = np.zeros(rows, np.int32) if diagnostics else None
+        ct = np.zeros(rows, np.float32) if diagnostics else None
+        cl = np.zeros(rows, np.float32) if diagnostics else None
+        cj = np.zeros(rows, np.float32) if diagnostics else None
+        fb = np.zeros(n, np.int32) if diagnostics else None
+        ctb = np.zeros(n, np.float32) if diagnostics else None
+        clb = np.zeros(n, np.float32) if diagnostics else None
+        cjb = np.zeros(n, np.float32) if diagnostics else None
+        rc = lib().orc_bm_run_f32(C.byref(sc), _p(sec), _p(gamma), _p(items), C.c_int(n), _p(inp),
+                                  C.c_int64(inp.size), _p(speeds), C.c_int64(rows), _p(fn), _p(ct), _p(cl),
+                                  _p(cj), _p(fb), _p(ctb), _p(clb), _p(cjb), _p(steps), _p(passes),
+                                  _p(sweeps), C.c_int(threads))
+        out.update(speeds=speeds, failed_nodes=fn, conv_time=ct, conv_loops=cl, conv_jacobi=cj,
+                   failed_blocks=fb, conv_time_blocks=ctb, conv_loops_blocks=clb, conv_jacobi_blocks=cjb)
+    else:
+        speeds = np.zeros((rows, SECTIONS), np.float64)
+        rc = lib().orc_bm_run_f64(C.byref(sc), _p(sec), _p(gamma), _p(items), C.c_int(n), _p(inp),
+                                  C.c_int64(inp.size), _p(speeds), C.c_int64(rows), _p(steps), _p(passes),
+                                  _p(sweeps), C.c_int(threads))
+        out.update(speeds=speeds)
+    assert rc == 0, rc
+    return out
+
+
+def jacobi(Y: np.ndarray, rsm: np.ndarray, p0: np.ndarray, sweeps: int) -> np.ndarray:
+    p = np.ascontiguousarray(p0, np.float32).copy()
+    lib().orc_jacobi(_p(np.ascontiguousarray(Y, np.float32)), _p(np.ascontiguousarray(rsm, np.float32)),
+                     _p(p), C.c_int(sweeps))
+    return p
+
+
+def ihc_lambda(bm: np.ndarray, interval_len: int, frdp: int, ac_b: np.ndarray, dc_b: np.ndarray,
+               ihc_damage: np.ndarray, A: np.ndarray, spont: np.ndarray, lambda_sat: float = 500.0,
+               eta_ac: float = 1.0, eta_dc: float = 100.0, scale_bm: float = 1.0, stages: bool = False,
+               threads: int = 0):
+    bm = np.ascontiguousarray(bm, np.float32)
+    rows = bm.shape[0]
+    lam = np.zeros((LAMBDA_COUNT, rows, SECTIONS), np.float32)
+    st = np.zeros((5, rows, SECTIONS), np.float32) if stages else None
+    ac_b = np.ascontiguousarray(ac_b, np.float64)
+    dc_b = np.ascontiguousarray(dc_b, np.float64)
+    rc = lib().orc_ihc_lambda(_p(bm), C.c_int64(rows), C.c_int(interval_len), C.c_int(frdp), _p(ac_b),
+                              C.c_int(ac_b.size), _p(dc_b), C.c_int(dc_b.size),
+                              _p(np.ascontiguousarray(ihc_damage, np.float64)),
+                              _p(np.ascontiguousarray(A, np.float32)), _p(np.ascontiguousarray(spont, np.float32)),
+                              C.c_float(lambda_sat), C.c_float(eta_ac), C.c_float(eta_dc), C.c_float(scale_bm),
+                              _p(lam), _p(st), C.c_int(threads))
+    assert rc == 0
+    return (lam, st) if stages else lam
+
+
+def jnd(lam: np.ndarray, n_intervals: int, interval_nodes: int, head_nodes: int, length_nodes: int,
+        dA: np.ndarray, ref_index: np.ndarray, nIHC: np.ndarray, W: np.ndarray, Fs: float,
+        delta_alpha_factor: float, calc_ai: bool = True, calc_rms: bool = True):
+    lam = np.ascontiguousarray(lam, np.float32)
+    rows = lam.shape[1]
+    mr = np.zeros((LAMBDA_COUNT, n_intervals, SECTIONS), np.float32)
+    fra = np.zeros((LAMBDA_COUNT, n_intervals), np.float32)
+    fai = np.zeros((LAMBDA_COUNT, n_intervals), np.float32)
+    rc = lib().orc_jnd(_p(lam), C.c_int64(rows), C.c_int(n_intervals), C.c_int(interval_nodes),
+                       C.c_int(head_nodes), C.c_int(length_nodes), _p(np.ascontiguousarray(dA, np.float32)),
+                       _p(np.ascontiguousarray(ref_index, np.int32)), _p(np.ascontiguousarray(nIHC, np.float64)),
+                       _p(np.ascontiguousarray(W, np.float32)), C.c_double(Fs), C.c_double(delta_alpha_factor),
+                       C.c_int(int(calc_ai)), C.c_int(int(calc_rms)), _p(mr), _p(fra), _p(fai))
+    assert rc == 0
+    return mr, fra, fai
+
+
+def generate_input(dA, Wn, freq, WN, wn_dc, wn_norm, sig_norm, interval_len, head, Fs):
+    n = len(dA)
+    out = np.zeros(n * interval_len, np.float32)
+    lib().orc_generate_input(C.c_int(n), _p(np.ascontiguousarray(dA, np.float64)),
+                             _p(np.ascontiguousarray(Wn, np.float64)), _p(np.ascontiguousarray(freq, np.float64)),
+                             _p(np.ascontiguousarray(WN, np.float32)), C.c_double(wn_dc), C.c_double(wn_norm),
+                             C.c_double(sig_norm), C.c_int(interval_len), C.c_int(head), C.c_float(Fs), _p(out))
+    return out
