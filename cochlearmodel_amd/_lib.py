@@ -30,6 +30,9 @@ ITEM_DTYPE = np.dtype([("start", np.int32), ("mosi", np.int32), ("mesi", np.int3
                        ("write_start", np.int32), ("tol", np.float32), ("m1tol", np.float32),
                        ("set", np.int32), ("pad", np.int32)])
 STATS_DTYPE = np.dtype([("steps", np.int64), ("passes", np.int64), ("sweeps", np.int64)])
+IHC_INTERVAL_DTYPE = np.dtype([("row0", np.int64), ("len", np.int32), ("kbase", np.int32), ("L", np.int32),
+                               ("tla", np.int32), ("frdp", np.int32), ("set", np.int32)])
+assert IHC_INTERVAL_DTYPE.itemsize == 32
 PROFILE_DTYPE = np.dtype([("dA", np.float64), ("recipdA", np.float64), ("dBSPLSignal", np.float64),
                           ("dBSPLNoise", np.float64), ("Wn", np.float64), ("isReference", np.int8),
                           ("pad0", np.int8, 3), ("calculateIndexPosition", np.int32),
@@ -73,6 +76,41 @@ _SIGS = {
     "GeneralKernel_Copy_Results_Double": (C.c_int, [vp, vp, sz]),
     "setupToleranceProfile": (C.c_int, [vp, C.c_int, f32, f32, C.c_int, f32, f32, C.c_int, C.c_int, C.c_int]),
     "BMOHCKernel_Free": (C.c_int, []),
+    "IHCNewKernel": (C.c_int, [vp, vp, vp, C.c_int, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, f64, vp, C.c_int,
+                               C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, f32, f32, f32, C.c_int, C.c_int,
+                               C.c_int, C.c_int, C.c_int, C.c_int, f32, C.c_int, C.c_int, C.c_int, C.c_int,
+                               C.c_int]),
+    "loadAihc": (C.c_int, [vp]),
+    "enableloadAihc": (C.c_int, []),
+    "allocateBuffer": (C.c_int, [C.c_int, C.c_int]),
+    "releaseBuffer": (C.c_int, [C.c_int]),
+    "RunIHCKernel": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "BMOHCKernel_Copy_Lambda": (C.c_int, [vp, sz, C.c_int]),
+    "getCudaLambda": (vp, []),
+    "getCudaBuffer": (vp, []),
+    "getCudaMeanRate": (vp, []),
+    "cm_ihc_lambda_run": (C.c_int, [vp, i64, vp, C.c_int, vp, C.c_int, vp, vp, C.c_int, vp, C.c_int, vp, vp, vp, f32,
+                                    f32, f32, f32, vp, vp, vp]),
+    "CudaCalculateJND": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, f64, f64, f64, vp, vp, C.c_int, vp, C.c_int,
+                                   C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp,
+                                   C.c_int, C.c_int, f64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "cm_jnd_run": (C.c_int, [vp, i64, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, vp, C.c_int, vp, vp, f64, f64,
+                             C.c_int, C.c_int, vp, vp, vp]),
+    "cm_jnd_final": (C.c_int, [vp, vp, C.c_int, C.c_int, C.c_int, vp, C.c_int, vp, f64, f64, C.c_int, C.c_int, vp,
+                               vp, vp, vp]),
+    "InputProfilesArrayInitialization": (C.c_int, [C.c_int] * 5),
+    "InputProfilesArrayTermination": (C.c_int, []),
+    "uploadProfiles": (C.c_int, [vp, C.c_int, C.c_int]),
+    "cm_input_normalization": (C.c_int, [vp, C.c_int, vp, C.c_int, C.c_int, C.c_int, C.c_int, f64, f64, C.c_int,
+                                         C.c_int, C.c_int, C.c_int, f32, vp]),
+    "generateInputFromProfile": (C.c_int, [vp, vp, C.c_int, vp, C.c_int, C.c_int, C.c_int, C.c_int, f64, f64,
+                                           C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                           C.c_int, C.c_int, C.c_int, C.c_int, f32, C.c_int, vp, C.c_int, C.c_int,
+                                           f32, f32, C.c_int, f32, f32, C.c_int, C.c_int, vp, C.c_int, vp,
+                                           C.c_int]),
+    "cm_generate_input_run": (C.c_int, [vp, vp, vp, vp, f64, f64, C.c_int, f64, f64, C.c_int, C.c_int, C.c_int,
+                                        C.c_int, f32]),
+    "IHCKernel_Free": (C.c_int, []),
     "cm_model_vectors": (C.c_int, [vp]),
     "cm_tri_matrix": (C.c_int, [vp, vp, vp]),
     "cm_pack_section_params": (C.c_int, [vp]),

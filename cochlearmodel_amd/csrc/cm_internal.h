@@ -28,6 +28,28 @@ struct BmLaunch {
   cm_item_stats *stats;
 };
 
+struct JndArgs {
+  const float *lambda;      // 3 x rows x 256
+  int64_t rows;             // plane stride
+  int n_intervals;
+  int interval_nodes;
+  int head_nodes;
+  int length_nodes;
+  const float *dA;          // per local interval
+  const int32_t *ref_index; // per local interval (itself for references)
+  const int32_t *out_index; // per local interval: column in F_RA / FisherAISum
+  int out_stride;           // columns per fiber group in F_RA / FisherAISum
+  const double *nIHC;       // 256
+  float W[3];
+  double Fs;
+  float T;                  // (float)(delta_alpha_factor / Fs)
+  float Tlength;            // (float)(float(len) / Fs)
+  int calc_ai, calc_rms;
+  float *mean_rate;         // 3 x n_intervals x 256
+  float *F_RA;
+  float *FisherAISum;
+};
+
 void set_error(const char *fmt, ...);
 int check(hipError_t e, const char *what);
 hipStream_t stream();
@@ -40,5 +62,21 @@ int bm_ensure_input(size_t n);
 }  // namespace cm
 
 extern "C" int cm_launch_bm(const cm::BmLaunch *a, int waves_per_block, int grid_blocks, hipStream_t stream);
+extern "C" int cm_launch_ihc_lambda(const float *d_bm, int64_t rows, const cm_ihc_interval *d_ivs,
+                                    const cm_ihc_interval *h_ivs, int n_ivs, const cm_ihc_interval *d_iir_ivs,
+                                    int n_iir_ivs, const double *d_ac_b, const double *d_ac_a, int n_ac,
+                                    const double *d_dc, int n_dc, const double *d_damage, const float *d_A,
+                                    const float *spont3, float lambda_sat, float eta_ac, float eta_dc, float scale_bm,
+                                    float *d_scratch, float *d_lambda, float *d_stages, hipStream_t stream);
+extern "C" int cm_launch_jnd(const cm::JndArgs *a, hipStream_t stream);
+extern "C" int cm_launch_calc_dA(const float *input, float *dA, int n, int interval_nodes, int head_nodes,
+                                 int length_nodes, int offset_start, hipStream_t stream);
+extern "C" int cm_launch_generate_input(const cm_jnd_profile *d_prof, const float *d_WN, const float *d_Signal,
+                                        float *d_out, double wn_dc, double wn_norm, int signal_mode, double sig_dc,
+                                        double sig_norm, int start_profile, int n_profiles, int interval_len,
+                                        int head, float Fs, hipStream_t stream);
+extern "C" int cm_launch_hearing_aid(const float *d_in, float *d_tmp, float *d_out, int n_profiles, int interval_len,
+                                     const float *d_fir, int fir_n, const float *d_iir, int iir_n,
+                                     hipStream_t stream);
 
 #endif

@@ -56,6 +56,19 @@ typedef struct {
   int64_t sweeps;  // Jacobi sweeps
 } cm_item_stats;
 
+// A run of rows of the IHC/lambda stage sharing one FIR window rule:
+// k = kbase + (row - row0); taps used = min((k+1) mod per, taps) with
+// per = L when k < frdp-1 and per = tla afterwards (src/cochlea.cu:2719-2728).
+typedef struct {
+  int64_t row0;   // first row in the buffer
+  int32_t len;    // rows in the interval
+  int32_t kbase;  // k of row0
+  int32_t L;      // interval length before the final regular division
+  int32_t tla;    // analysed length after it
+  int32_t frdp;   // final regular division position
+  int32_t set;    // IHC damage vector index
+} cm_ihc_interval;
+
 #ifdef __cplusplus
 }
 #endif

@@ -201,6 +201,10 @@ int cm_set_bm_launch(int waves_per_block, int persistent) {
     set_error("waves_per_block must be 1, 2 or 4 (got %d)", waves_per_block);
     return -1;
   }
+  if (persistent) {
+    set_error("cm_set_bm_launch: the persistent work-queue launch is disabled");
+    return -3;
+  }
   g_bm.waves_per_block = waves_per_block;
   g_bm.persistent = persistent;
   return 0;
@@ -359,6 +363,20 @@ int BMOHCNewKernel(const float *input_samples, int override_input_samples, float
             "upload blocks"))
     return -1;
   g_bm.last_blocks = Time_Blocks;
+  // rows no block owns (before the first block's output, after the last block's
+  // end) are zero rather than left over from a previous run
+  {
+    const size_t first = (size_t)std::max(0, items[0].write_start);
+    const size_t last = (size_t)std::min(samplesBufferLengthP1, items[Time_Blocks - 1].mesi);
+    if (first > 0 && check(hipMemsetAsync(g_bm.speeds.ptr, 0, first * CM_SECTIONS * sizeof(float), stream()),
+                           "zero leading rows"))
+      return -1;
+    if ((size_t)samplesBufferLengthP1 > last &&
+        check(hipMemsetAsync(g_bm.speeds.ptr + last * CM_SECTIONS, 0,
+                             ((size_t)samplesBufferLengthP1 - last) * CM_SECTIONS * sizeof(float), stream()),
+              "zero trailing rows"))
+      return -1;
+  }
   float ms = 0.f;
   int rc = launch_bm_items(sc, g_bm.input.ptr, (int64_t)samplesBufferLengthP1 + 2, g_bm.sec.ptr, g_bm.gamma.ptr,
                            g_bm.items.ptr, Time_Blocks, g_bm.speeds.ptr, (int64_t)samplesBufferLengthP1,
@@ -384,6 +402,12 @@ int cm_bm_run(const cm_bm_scalars *sc, const float *d_input, int64_t input_len, 
   if (!sc || !d_input || !d_sec || !d_gamma || !d_items || n_items < 0) {
     set_error("cm_bm_run: null argument");
     return -1;
+  }
+  if (persistent) {
+    // The work-queue variant went silent on its first MI355X run and its cause
+    // has not been shown; it stays switched off.
+    set_error("cm_bm_run: the persistent work-queue launch is disabled");
+    return -3;
   }
   if (sc->j_fast < 0 || sc->j_slow < 0 || sc->max_passes < 1) {
     set_error("cm_bm_run: invalid Jacobi/pass counts");

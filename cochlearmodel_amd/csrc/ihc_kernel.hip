@@ -1,0 +1,179 @@
+// IHC and auditory-nerve rates (lambda) of the reference's RunIHCKernel
+// (src/cochlea.cu:3573-3644): AC filter of BM velocity (FIR in
+// CudaUnifiedCalcIHC :3431-3506 or IIR in CudaIIRFilter[Decoupled]
+// :2840-2908), dS = (BM - AC)^2, box-car DC FIR, PRE_IHC = eta_AC*AC +
+// eta_DC*DC, IHC = log10(max(10^IHC_Vector * PRE_IHC, 0) + 1e-20) and
+// lambda_k = min(SAT, max(A_k * IHC, spont_k)) with section 0 at spont
+// (CudaCalcLambda :3365-3408).
+//
+// Data layout: rows x 256 sections, section fastest; a workgroup is the 256
+// sections of a chunk of rows, so every load and store is a coalesced 1 KB
+// row.  Work is described by an interval table: each interval is a run of
+// rows whose FIR window follows min((k+1) mod per, taps) with the
+// reference's per = L before the final regular division and the analysed
+// length after it (src/cochlea.cu:2719-2728).  FIR sums accumulate in FP64
+// in tap order, as the reference does, so the CPU oracle agrees bit for bit.
+#include <hip/hip_runtime.h>
+
+#include "cm_internal.h"
+
+namespace cm {
+
+__device__ __forceinline__ int fir_window(int64_t k, const cm_ihc_interval &iv, int taps) {
+  const int64_t sg = iv.frdp > 1 ? k / (iv.frdp - 1) : 1;
+  const int64_t per = sg * iv.tla + (1 - sg) * iv.L;
+  if (per <= 0) return 0;
+  const int64_t w = (k + 1) % per;
+  return (int)(w < taps ? w : taps);
+}
+
+// AC (FIR, or read back when an IIR pass already wrote it) and dS.
+__global__ __launch_bounds__(256) void ac_ds_kernel(const float *__restrict__ bm, const cm_ihc_interval *__restrict__ ivs,
+                                                    const double *__restrict__ b, int nb, float coefA, double coefB,
+                                                    int rows_per_block, int ac_ready, float *__restrict__ AC,
+                                                    float *__restrict__ dS) {
+  const cm_ihc_interval iv = ivs[blockIdx.y];
+  const int s = threadIdx.x;
+  const int64_t r0 = (int64_t)blockIdx.x * rows_per_block;
+  if (r0 >= iv.len) return;
+  const int64_t r1 = r0 + rows_per_block < iv.len ? r0 + rows_per_block : iv.len;
+  for (int64_t r = r0; r < r1; ++r) {
+    const int64_t row = iv.row0 + r;
+    const int64_t k = iv.kbase + r;
+    float ac;
+    if (ac_ready) {
+      ac = AC[row * CM_SECTIONS + s];
+    } else {
+      const int w = fir_window(k, iv, nb);
+      double y = 0.0;
+      for (int i = 0; i < w; ++i) y = y + b[i] * (double)bm[(row - i) * CM_SECTIONS + s];
+      ac = (float)y;
+      AC[row * CM_SECTIONS + s] = ac;
+    }
+    const float mid = __builtin_fmaf(bm[row * CM_SECTIONS + s], coefA, (float)((double)ac * coefB));
+    dS[row * CM_SECTIONS + s] = mid * mid;
+  }
+}
+
+// IIR AC filter, one thread per section, serial over the interval
+// (DeviceCudaIIRFilter<float,float>, src/cochlea.cu:2742-2772).
+__global__ __launch_bounds__(256) void ac_iir_kernel(const float *__restrict__ bm, const cm_ihc_interval *__restrict__ ivs,
+                                                     const double *__restrict__ b, const double *__restrict__ a,
+                                                     int taps, float *__restrict__ AC) {
+  const cm_ihc_interval iv = ivs[blockIdx.x];
+  const int s = threadIdx.x;
+  const int64_t per = iv.len;
+  for (int64_t k = 0; k < iv.len; ++k) {
+    const int64_t row = iv.row0 + k;
+    const int ob = per > 0 ? (int)(((k + 1) % per) < taps ? ((k + 1) % per) : taps) : 0;
+    float y = 0.0f;
+    for (int i = 0; i < ob; ++i) y = (float)((double)y + b[i] * (double)bm[(row - i) * CM_SECTIONS + s]);
+    for (int i = 0; i < ob - 1; ++i) y = (float)((double)y - a[i + 1] * (double)AC[(row - i - 1) * CM_SECTIONS + s]);
+    AC[row * CM_SECTIONS + s] = y;
+  }
+}
+
+struct LambdaArgs {
+  const float *AC;
+  const float *dS;
+  const cm_ihc_interval *ivs;
+  const double *dc;
+  int ndc;
+  float etaAC;
+  double etaDC;
+  const double *damage;   // sets x 256
+  const float *A;         // 3 x 256
+  float spont[3];
+  double sat;
+  int64_t rows;           // lambda plane stride in rows
+  float *lambda;          // 3 x rows x 256
+  float *stages;          // optional 5 x rows x 256 (AC, dS, DC, PRE_IHC, IHC)
+  int rows_per_block;
+};
+
+// DC box-car FIR, PRE_IHC, IHC and the three lambdas.
+__global__ __launch_bounds__(256) void dc_lambda_kernel(LambdaArgs a) {
+  const cm_ihc_interval iv = a.ivs[blockIdx.y];
+  const int s = threadIdx.x;
+  const int64_t r0 = (int64_t)blockIdx.x * a.rows_per_block;
+  if (r0 >= iv.len) return;
+  const int64_t r1 = r0 + a.rows_per_block < iv.len ? r0 + a.rows_per_block : iv.len;
+  const double dmg = a.damage[(int64_t)iv.set * CM_SECTIONS + s];
+  const double zero_factor = s != 0 ? 1.0 : 0.0;
+  double A3[3], sp[3], zo[3];
+  for (int j = 0; j < 3; ++j) {
+    A3[j] = (double)a.A[j * CM_SECTIONS + s];
+    sp[j] = (double)a.spont[j];
+    zo[j] = s != 0 ? 0.0 : sp[j];
+  }
+  const int64_t plane = a.rows * CM_SECTIONS;
+  for (int64_t r = r0; r < r1; ++r) {
+    const int64_t row = iv.row0 + r;
+    const int64_t k = iv.kbase + r;
+    const int w = fir_window(k, iv, a.ndc);
+    double y = 0.0;
+    for (int i = 0; i < w; ++i) y = y + a.dc[i] * (double)a.dS[(row - i) * CM_SECTIONS + s];
+    const float dc = (float)y;
+    const int64_t o = row * CM_SECTIONS + s;
+    const float ac = a.AC[o];
+    const float pre = __builtin_fmaf(ac, a.etaAC, (float)((double)dc * a.etaDC));
+    const float ihc = (float)log10(fmax((double)(dmg * (double)pre), 0.0) + 1e-20);
+    for (int j = 0; j < 3; ++j) {
+      const double base = fmin(a.sat, fmax(A3[j] * (double)ihc, sp[j]));
+      a.lambda[j * plane + o] = (float)fma(zero_factor, base, zo[j]);
+    }
+    if (a.stages) {
+      a.stages[0 * plane + o] = ac;
+      a.stages[1 * plane + o] = a.dS[o];
+      a.stages[2 * plane + o] = dc;
+      a.stages[3 * plane + o] = pre;
+      a.stages[4 * plane + o] = ihc;
+    }
+  }
+}
+
+}  // namespace cm
+
+// Runs the whole IHC/lambda chain for one interval table.  scratch must hold
+// 2 x rows x 256 floats (AC, dS).
+extern "C" int cm_launch_ihc_lambda(const float *d_bm, int64_t rows, const cm_ihc_interval *d_ivs,
+                                    const cm_ihc_interval *h_ivs, int n_ivs, const cm_ihc_interval *d_iir_ivs,
+                                    int n_iir_ivs, const double *d_ac_b, const double *d_ac_a, int n_ac,
+                                    const double *d_dc, int n_dc, const double *d_damage, const float *d_A,
+                                    const float *spont3, float lambda_sat, float eta_ac, float eta_dc, float scale_bm,
+                                    float *d_scratch, float *d_lambda, float *d_stages, hipStream_t stream) {
+  using namespace cm;
+  if (n_ivs <= 0) return 0;
+  int64_t max_len = 0;
+  for (int i = 0; i < n_ivs; ++i) max_len = h_ivs[i].len > max_len ? h_ivs[i].len : max_len;
+  const int rpb = 32;
+  const int64_t gx = (max_len + rpb - 1) / rpb;
+  if (gx > 0x7fffffff || n_ivs > 65535) return -3;
+  float *AC = d_scratch;
+  float *dS = d_scratch + rows * CM_SECTIONS;
+  const int iir = d_ac_a != nullptr;
+  if (iir) {
+    hipLaunchKernelGGL(ac_iir_kernel, dim3(n_iir_ivs), dim3(256), 0, stream, d_bm, d_iir_ivs, d_ac_b, d_ac_a, n_ac,
+                       AC);
+  }
+  hipLaunchKernelGGL(ac_ds_kernel, dim3((unsigned)gx, n_ivs), dim3(256), 0, stream, d_bm, d_ivs, d_ac_b, n_ac,
+                     scale_bm, -1.0, rpb, iir, AC, dS);
+  LambdaArgs a;
+  a.AC = AC;
+  a.dS = dS;
+  a.ivs = d_ivs;
+  a.dc = d_dc;
+  a.ndc = n_dc;
+  a.etaAC = eta_ac;
+  a.etaDC = (double)eta_dc;
+  a.damage = d_damage;
+  a.A = d_A;
+  for (int j = 0; j < 3; ++j) a.spont[j] = spont3[j];
+  a.sat = (double)lambda_sat;
+  a.rows = rows;
+  a.lambda = d_lambda;
+  a.stages = d_stages;
+  a.rows_per_block = rpb;
+  hipLaunchKernelGGL(dc_lambda_kernel, dim3((unsigned)gx, n_ivs), dim3(256), 0, stream, a);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}

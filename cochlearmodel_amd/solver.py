@@ -82,6 +82,75 @@ def voice_layout(Fs: float, time_blocks: int, time_block_length: float, overlap_
     return VoiceLayout(Fs, time_blocks, block, overlap)
 
 
+def block_table(time_blocks: int, bufP1: int, overlap: int, decouple_filter: int) -> np.ndarray:
+    """Time-block table of BMOHCNewKernel (src/cochlea.cu:1148-1176) as computed
+    by the library for a launch; tolerances left at zero."""
+    items = np.zeros(time_blocks, ITEM_DTYPE)
+    I8 = (bufP1 - overlap) // (time_blocks + 1)
+    DF = decouple_filter
+    prev_end = 0
+    for b in range(time_blocks):
+        transient = int(b > 0 and (DF == 0 or (DF != 1 and b % DF != 0)))
+        pre = int(DF > 0 and (DF == 1 or (b + 1) % DF == 0) and b != time_blocks - 1)
+        post = int(DF > 0 and b % DF == 0)
+        in0 = b * I8
+        so = in0 + max(1 - transient - post, 0) * overlap
+        eo = in0 + I8 + (1 - pre) * overlap
+        if b > 0:
+            so = max(so, prev_end)
+        first = max(in0, so - 1)
+        if b > 0 and first < prev_end:
+            first = prev_end
+        items[b] = (in0, so, eo, first, 0.0, 0.0, 0, 0)
+        prev_end = eo
+    return items
+
+
+class BatchedBM:
+    """Device-resident batched BM launches through cm_bm_run (torch tensors are
+    only the memory/stream plumbing)."""
+
+    def __init__(self, sets_gamma: np.ndarray, device: str = "cuda"):
+        import torch
+        self.torch = torch
+        self.device = device
+        self.sec = torch.from_numpy(section_params().reshape(-1)).to(device)
+        self.gamma = torch.from_numpy(np.ascontiguousarray(np.atleast_2d(sets_gamma), np.float32).reshape(-1)).to(
+            device)
+
+    def run(self, sc: BMScalars, d_input, items: np.ndarray, rows: int, waves_per_block: int = 1,
+            persistent: int = 0, timed: bool = False, diagnostics: bool = False, d_speeds=None):
+        torch = self.torch
+        dev = self.device
+        items_d = torch.from_numpy(np.ascontiguousarray(items).view(np.uint8).copy()).to(dev)
+        n = len(items)
+        if d_speeds is None:
+            d_speeds = torch.zeros((rows, SECTIONS), dtype=torch.float32, device=dev)
+        stats = torch.zeros(n * 3, dtype=torch.int64, device=dev)
+        diag = {}
+        if diagnostics:
+            diag = dict(failed_nodes=torch.zeros(rows, dtype=torch.int32, device=dev),
+                        conv_time=torch.zeros(rows, dtype=torch.float32, device=dev),
+                        conv_loops=torch.zeros(rows, dtype=torch.float32, device=dev),
+                        conv_jacobi=torch.zeros(rows, dtype=torch.float32, device=dev),
+                        failed_blocks=torch.zeros(n, dtype=torch.int32, device=dev),
+                        conv_time_blocks=torch.zeros(n, dtype=torch.float32, device=dev),
+                        conv_loops_blocks=torch.zeros(n, dtype=torch.float32, device=dev),
+                        conv_jacobi_blocks=torch.zeros(n, dtype=torch.float32, device=dev))
+        g = lambda k: ptr(diag.get(k))
+        L = lib()
+        L.cm_set_stream(C.c_void_p(torch.cuda.current_stream().cuda_stream))
+        rc = L.cm_bm_run(C.byref(sc), ptr(d_input), d_input.numel(), ptr(self.sec), ptr(self.gamma), ptr(items_d), n,
+                         ptr(d_speeds), rows, g("failed_nodes"), g("conv_time"), g("conv_loops"), g("conv_jacobi"),
+                         g("failed_blocks"), g("conv_time_blocks"), g("conv_loops_blocks"), g("conv_jacobi_blocks"),
+                         ptr(stats), waves_per_block, persistent, 1 if timed else 0)
+        check(rc, "cm_bm_run")
+        out = {"speeds": d_speeds, "stats": stats.view(n, 3), **diag, "_items": items_d}
+        if timed:
+            out["kernel_ms"] = L.cm_last_kernel_ms()
+        return out
+
+
 def run_bm_voice(signal: np.ndarray, layout: VoiceLayout, ohc=0.5, max_m1_sp_error=1e-15, max_tolerance=1e-10,
                  relative_error=0, m1_fix=0.0, tol_fix=0.0, spl_ref=2e-5, j_fast=2, j_slow=20, max_passes=10,
                  decouple_filter=0, timed=False):

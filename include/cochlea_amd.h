@@ -115,6 +115,97 @@ int setupToleranceProfile(const cm_jnd_profile *profiles, int is_first_time_for_
 /* replaces BMOHCKernel_Free (src/cochlea.cu:815-822) */
 int BMOHCKernel_Free(void);
 
+/* ---- IHC / lambda ------------------------------------------------------ */
+/* replaces IHCNewKernel, include/cochlea_common.h:238-274 (src/cochlea.cu:2417-2598);
+ * Nerves_Clusters = {spont[3], unused[3], W[3]}; lambda_offset must be 0 */
+int IHCNewKernel(const double *IHC_Damage_Factor, const float *Nerves_Clusters, const double *dc_filter,
+                 int order_of_dc_filter, const double *ac_b_filter, const double *ac_a_filter, int is_ac_iir_filter,
+                 int order_of_ac_filter, int cochlea_sections, int time_blocks, double SPLRefVal,
+                 const float *backup_speeds, int backup_speeds_length, int calcTime, int writeTime, int allocateTime,
+                 int intervalTimeNodes, int max_backup_nodes_len, int lambda_offset, float Lambda_SAT, float eta_AC,
+                 float eta_DC, int first_time, int first_time_for_param_set, int loadedFromHD,
+                 int disable_advanced_memory_handling, int review_memory_handling, int asMemoryHandlingOnly,
+                 float scaleBMVelocityForLambdaCalculation, int CalculateJNDOnGPU, int maxJNDIntervals,
+                 int overlapNodes, int Decouple_Filter, int Show_Run_Time);
+/* replace loadAihc / enableloadAihc (src/cochlea.cu:449-455): 3 x 256 floats */
+int loadAihc(const float *Aihc);
+int enableloadAihc(void);
+/* replace allocateBuffer / releaseBuffer (src/cochlea.cu:2599-2628) */
+int allocateBuffer(int size_in_nodes, int Show_Run_Time);
+int releaseBuffer(int Show_Run_Time);
+/* replaces RunIHCKernel, include/cochlea_common.h:276 (src/cochlea.cu:3573-3644);
+ * the IHC stage is not written over the BM velocity buffer */
+int RunIHCKernel(float *host_backup, int Show_Run_Time, int save_lambda, int backup_stage,
+                 int Decouple_Unified_IHC_Factor);
+/* replaces BMOHCKernel_Copy_Lambda (src/cochlea.cu:834-837) */
+int BMOHCKernel_Copy_Lambda(float *target, size_t lambdaNodes, int offset);
+/* replace getCudaLambda / getCudaBuffer / getCudaMeanRate */
+void *getCudaLambda(void);
+void *getCudaBuffer(void);
+void *getCudaMeanRate(void);
+/* batched IHC/lambda over device pointers and an interval table (no
+ * reference counterpart); d_scratch holds 2 x rows x 256 floats */
+int cm_ihc_lambda_run(const float *d_bm, int64_t rows, const cm_ihc_interval *h_ivs, int n_ivs,
+                      const cm_ihc_interval *h_iir_ivs, int n_iir_ivs, const double *d_ac_b, const double *d_ac_a,
+                      int n_ac, const double *d_dc, int n_dc, const double *d_damage, const float *d_A,
+                      const float *spont3, float lambda_sat, float eta_ac, float eta_dc, float scale_bm,
+                      float *d_scratch, float *d_lambda, float *d_stages);
+
+/* ---- JND --------------------------------------------------------------- */
+/* replaces CudaCalculateJND, include/cochlea_common.h:292-328 (src/cochlea.cu:3946-4065) */
+int CudaCalculateJND(int calculate_ai, int calculate_rms, int mean_size, int fisher_size, double SPLRefVal, double Fs,
+                     double scaleBMVelocityForLambdaCalculation, const double *nIHC,
+                     const int *JND_Calculated_Intervals, int numOFJNDCalculated, const int *JND_Refrence_Intervals,
+                     int numOFJNDReferences, int handeledIntervalsJND, int JNDIntervalsFull, int JNDIntervals,
+                     int JNDIntervalHeadNodes, int overlapNodes, int JNDIntervalNodes, int lengthOffset,
+                     const int *JND_Serial_Intervals_Positions, const int *JND_Interval_To_Reference, float *F_RA,
+                     float *FisherAISum, int writeMatrixSize, int calculatedMatrixSize,
+                     double JND_Delta_Alpha_Length_Factor, int JND_USE_Spont_Base_Reference_Lambda,
+                     int Show_Run_Time, int calcdA, int show_generated_input_params_cuda, int backup_stage);
+/* batched JND over device pointers (no reference counterpart) */
+int cm_jnd_run(const float *d_lambda, int64_t rows, int n_intervals, int interval_nodes, int head_nodes,
+               int length_nodes, const float *d_dA, const int32_t *d_ref_index, const int32_t *d_out_index,
+               int out_stride, const double *d_nIHC, const float *W3, double Fs, double delta_alpha_factor,
+               int calc_ai, int calc_rms, float *d_mean_rate, float *d_F_RA, float *d_FisherAISum);
+/* replaces AudioGramCreator::calcJNDFinal (src/AudioGramCreator.cpp:454-508), host FP64 */
+int cm_jnd_final(const float *F_RA, const float *FisherAISum, int JNDIntervalsFull, int JNDIntervals,
+                 int handled, const int *refs, int nrefs, const int *serial, double SPLRefVal, double max_db,
+                 int calc_ai, int calc_rms, double *AiJND, double *RateJND, float *FisherAI, float *JND_RA);
+
+/* ---- input generation -------------------------------------------------- */
+/* replace InputProfilesArrayInitialization / Termination (src/cochlea.cu:2391-2415) */
+int InputProfilesArrayInitialization(int maxJNDIntervals, int wn_length, int signal_length, int signal_mode,
+                                     int Show_Generated_Input);
+int InputProfilesArrayTermination(void);
+/* replaces uploadProfiles (src/cochlea.cu:4288-4297) */
+int uploadProfiles(const cm_jnd_profile *profiles, int numOfProfiles, int profilesLoaded);
+/* the host normalisation of src/cochlea.cu:4165-4254: out4 = {wn_dc, wn_norm, sig_dc, sig_norm} */
+int cm_input_normalization(const float *WN, int wn_length, const float *Signal, int signal_length, int signal_mode,
+                           int Normalize_Sigma_Type, int Normalize_Sigma_Type_Signal, double given_interval,
+                           double remove_dc, int e0, int e1, int s0, int s1, float Fs, double *out4);
+/* replaces generateInputFromProfile, include/cochlea_common.h:373-413 (src/cochlea.cu:4298-4510);
+ * writes the BM input buffer */
+int generateInputFromProfile(const cm_jnd_profile *profiles, const float *WN, int wn_length, const float *Signal,
+                             int signal_length, int signal_mode, int Normalize_Sigma_Type,
+                             int Normalize_Sigma_Type_Signal, double Normalize_Noise_Energy_To_Given_Interval,
+                             double Remove_Generated_Noise_DC, int start_dc_expected_value_calculation,
+                             int end_dc_expected_value_calculation, int start_dc_normalized_value_calculation,
+                             int end_dc_normalized_value_calculation, int numOfProfiles, int profilesLoaded,
+                             int from_profile_index, int calculatedProfiles, int overlapNodes, int IntervalLength,
+                             int JND_Interval_Head, int JND_Interval_Actual_Length, float Fs,
+                             int Show_Generated_Input, float *target_input, int Show_Generated_Configuration,
+                             int is_first_time_for_parameters_set, float Max_M1_SP_Error_Parameter,
+                             float Max_Tolerance_Parameter, int Relative_Error_Parameters, float M1_SP_Fix_Factor,
+                             float Tolerance_Fix_Factor, int Blocks_Per_Interval, int Show_Run_Time,
+                             const float *fir_transfer_function, int fir_transfer_function_length,
+                             const float *iir_transfer_function, int iir_transfer_function_length);
+/* batched input synthesis over device pointers (no reference counterpart) */
+int cm_generate_input_run(const cm_jnd_profile *d_profiles, const float *d_WN, const float *d_Signal, float *d_out,
+                          double wn_dc, double wn_norm, int signal_mode, double sig_dc, double sig_norm,
+                          int start_profile, int n_profiles, int interval_len, int head, float Fs);
+/* replaces IHCKernel_Free (src/cochlea.cu:2629-2634) */
+int IHCKernel_Free(void);
+
 /* ---- host physical model ---------------------------------------------- */
 /* replaces CModel::CModel vectors (src/model.cpp:72-86): 7 x 256 doubles
  * M, R, S, R_tm, S_tm, S_ohc, Q */

@@ -477,10 +477,33 @@ static inline int fir_window(int64_t k, int64_t tla, int64_t L, int64_t frdp, in
   return (int)std::min<int64_t>(w, taps);
 }
 
-int orc_ihc_lambda(const float *bm, int64_t rows, int interval_len, int frdp, const double *ac_b, int n_ac,
-                   const double *dc_b, int n_dc, const double *ihc_damage, const float *A, const float *spont,
-                   float lambda_sat, float eta_ac, float eta_dc, float scale_bm, float *lambda, float *stages,
-                   int n_threads) {
+// IIR AC filter of DeviceCudaIIRFilter (src/cochlea.cu:2742-2772) run by
+// CudaIIRFilter (one interval over all rows, :2840-2857) or
+// CudaIIRFilterDecoupled (intervals of L rows, last one extended, :2885-2908).
+// Accumulation is in float with double products, as the <float,float> instance.
+static void iir_section(const float *bm, int64_t rows, int s, int64_t L, int64_t frdp, const double *b,
+                        const double *a, int taps, float *Y) {
+  const int64_t nint = L > 0 ? std::max<int64_t>(1, rows / L) : 1;
+  for (int64_t iv = 0; iv < nint; ++iv) {
+    const int64_t k0 = L > 0 ? iv * L : 0;
+    int64_t len = L > 0 ? L : rows;
+    if (iv + 1 == nint) len += rows - nint * len;
+    for (int64_t k = 0; k < len; ++k) {
+      int64_t sg = frdp > 1 ? k / (frdp - 1) : 1;
+      int64_t per = sg * len + (1 - sg) * len;
+      int ob = per > 0 ? (int)std::min<int64_t>((k + 1) % per, taps) : 0;
+      float y = 0.0f;
+      for (int i = 0; i < ob; ++i) y = (float)((double)y + b[i] * (double)bm[(k0 + k - i) * NS + s]);
+      for (int i = 0; i < ob - 1; ++i) y = (float)((double)y - a[i + 1] * (double)Y[k0 + k - i - 1]);
+      Y[k0 + k] = y;
+    }
+  }
+}
+
+int orc_ihc_lambda(const float *bm, int64_t rows, int interval_len, int frdp, const double *ac_b,
+                   const double *ac_a, int n_ac, const double *dc_b, int n_dc, const double *ihc_damage,
+                   const float *A, const float *spont, float lambda_sat, float eta_ac, float eta_dc, float scale_bm,
+                   float *lambda, float *stages, int n_threads) {
   if (!bm || !lambda || rows <= 0) return -1;
   const int64_t tla = rows;
   const int64_t L = interval_len > 0 ? interval_len : rows;
@@ -495,11 +518,15 @@ int orc_ihc_lambda(const float *bm, int64_t rows, int interval_len, int frdp, co
 #endif
   for (int s = 0; s < NS; ++s) {
     std::vector<float> AC(rows), dS(rows), DC(rows);
-    for (int64_t k = 0; k < rows; ++k) {   // AC FIR, Eq. 5.1 (double accumulate)
-      int w = fir_window(k, tla, L, frdp, n_ac);
-      double y = 0.0;
-      for (int i = 0; i < w; ++i) y = y + ac_b[i] * (double)bm[(k - i) * NS + s];
-      AC[k] = (float)y;
+    if (ac_a) {                            // AC IIR, Eq. 5.1
+      iir_section(bm, rows, s, interval_len > 0 ? interval_len : 0, frdp, ac_b, ac_a, n_ac, AC.data());
+    } else {
+      for (int64_t k = 0; k < rows; ++k) { // AC FIR, Eq. 5.1 (double accumulate)
+        int w = fir_window(k, tla, L, frdp, n_ac);
+        double y = 0.0;
+        for (int i = 0; i < w; ++i) y = y + ac_b[i] * (double)bm[(k - i) * NS + s];
+        AC[k] = (float)y;
+      }
     }
     for (int64_t k = 0; k < rows; ++k) {   // dS = (A*BM - AC)^2, Eq. 5.2
       float mid = fmaf(bm[k * NS + s], coefA, (float)((double)AC[k] * coefB));

@@ -97,7 +97,7 @@ float orc_expf(float x);
  * bm: rows x 256 (row-major, section fastest).  lambda: 3 x rows x 256.
  * stages (optional, may be NULL): 5 x rows x 256 = AC, dS, DC, PRE_IHC, IHC. */
 int orc_ihc_lambda(const float *bm, int64_t rows, int interval_len, int final_regular_division,
-                   const double *ac_b, int n_ac, const double *dc_b, int n_dc,
+                   const double *ac_b, const double *ac_a /* NULL: FIR */, int n_ac, const double *dc_b, int n_dc,
                    const double *ihc_damage, const float *A3x256, const float *spont3,
                    float lambda_sat, float eta_ac, float eta_dc, float scale_bm,
                    float *lambda, float *stages, int n_threads);

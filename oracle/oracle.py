@@ -160,14 +160,15 @@ def jacobi(Y: np.ndarray, rsm: np.ndarray, p0: np.ndarray, sweeps: int) -> np.nd
 def ihc_lambda(bm: np.ndarray, interval_len: int, frdp: int, ac_b: np.ndarray, dc_b: np.ndarray,
                ihc_damage: np.ndarray, A: np.ndarray, spont: np.ndarray, lambda_sat: float = 500.0,
                eta_ac: float = 1.0, eta_dc: float = 100.0, scale_bm: float = 1.0, stages: bool = False,
-               threads: int = 0):
+               threads: int = 0, ac_a: np.ndarray | None = None):
     bm = np.ascontiguousarray(bm, np.float32)
     rows = bm.shape[0]
     lam = np.zeros((LAMBDA_COUNT, rows, SECTIONS), np.float32)
     st = np.zeros((5, rows, SECTIONS), np.float32) if stages else None
     ac_b = np.ascontiguousarray(ac_b, np.float64)
     dc_b = np.ascontiguousarray(dc_b, np.float64)
-    rc = lib().orc_ihc_lambda(_p(bm), C.c_int64(rows), C.c_int(interval_len), C.c_int(frdp), _p(ac_b),
+    ac_a = None if ac_a is None else np.ascontiguousarray(ac_a, np.float64)
+    rc = lib().orc_ihc_lambda(_p(bm), C.c_int64(rows), C.c_int(interval_len), C.c_int(frdp), _p(ac_b), _p(ac_a),
                               C.c_int(ac_b.size), _p(dc_b), C.c_int(dc_b.size),
                               _p(np.ascontiguousarray(ihc_damage, np.float64)),
                               _p(np.ascontiguousarray(A, np.float32)), _p(np.ascontiguousarray(spont, np.float32)),

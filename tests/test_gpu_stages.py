@@ -1,0 +1,245 @@
+"""GPU parity of the IHC/lambda, JND and input-generation stages and of the
+batched BM launches against the CPU oracle, all through the C-ABI."""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+FS = 44100.0
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    n = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (n if n > 0 else 1.0)
+
+
+@pytest.fixture(scope="module")
+def c1_bm():
+    from cochlearmodel_amd import solver, workloads
+    lay = solver.voice_layout(FS, 5, 0.02, 0.005)
+    g = solver.run_bm_voice(workloads.c1_signal(), lay)
+    return lay, g
+
+
+def test_ihc_lambda_reference_sequence_fir(oracle, c1_bm):
+    from cochlearmodel_amd import stages
+    _say("case: IHC/lambda FIR on C1 BM, reference call sequence")
+    lay, g = c1_bm
+    rows = lay.total_nodes
+    lam, ihc = stages.run_ihc_reference(rows, lay.block_nodes, lay.time_blocks, 0, backup_stage=13)
+    b, _ = stages.load_ac_filter(FS)
+    lo, st = oracle.ihc_lambda(g["speeds"][:rows], rows, lay.block_nodes * lay.time_blocks, b, stages.dc_filter(FS),
+                               stages.ihc_damage(8.0), stages.aihc_table(),
+                               np.array(stages.SPONT_DEFAULT, np.float32), stages=True, threads=8)
+    _say(f"lambda rel L2 {_rel(lam, lo):.3e} exact {np.mean(lam == lo):.6f}; IHC exact {np.mean(ihc == st[4]):.6f}")
+    assert _rel(lam, lo) <= 1e-4
+    assert _rel(ihc, st[4]) <= 1e-4
+    assert np.all(lam[:, :, 0] == np.array(stages.SPONT_DEFAULT, np.float32)[:, None])
+
+
+def test_ihc_lambda_iir_decoupled(oracle, c1_bm):
+    from scipy.signal import butter
+    from cochlearmodel_amd import stages
+    _say("case: IHC/lambda IIR (Butterworth 4) with Decouple_Filter=1")
+    lay, g = c1_bm
+    rows = lay.total_nodes
+    b, a = butter(4, 1000 / 22050)
+    lam, _ = stages.run_ihc_reference(rows, lay.block_nodes, lay.time_blocks, 1, ac=(b, a))
+    lo = oracle.ihc_lambda(g["speeds"][:rows], lay.block_nodes, lay.block_nodes * lay.time_blocks, b,
+                           stages.dc_filter(FS), stages.ihc_damage(8.0), stages.aihc_table(),
+                           np.array(stages.SPONT_DEFAULT, np.float32), ac_a=a, threads=8)
+    _say(f"lambda rel L2 {_rel(lam, lo):.3e} exact {np.mean(lam == lo):.6f}")
+    assert _rel(lam, lo) <= 1e-4
+
+
+def test_ihc_batched_signal_intervals(oracle):
+    import torch
+    from cochlearmodel_amd import stages
+    _say("case: batched IHC over 3 uneven signals with per-signal IHC vectors")
+    rng = np.random.default_rng(11)
+    lens = [700, 1234, 401]
+    starts = np.cumsum([0] + lens[:-1])
+    rows = sum(lens)
+    bm = (1e-3 * rng.standard_normal((rows, 256))).astype(np.float32)
+    ihcv = [8.0, 6.5, 5.0]
+    bi = stages.BatchedIHC(FS, ihc_vectors=ihcv)
+    ivs = stages.signal_intervals(starts, lens, sets=[0, 1, 2])
+    d_lam, _ = bi.run(torch.from_numpy(bm).cuda(), rows, ivs)
+    lam = d_lam.cpu().numpy()
+    b, _ = stages.load_ac_filter(FS)
+    for k, (s0, n) in enumerate(zip(starts, lens)):
+        lo = oracle.ihc_lambda(bm[s0:s0 + n], n, n + 1, b, stages.dc_filter(FS), stages.ihc_damage(ihcv[k]),
+                               stages.aihc_table(), np.array(stages.SPONT_DEFAULT, np.float32), threads=8)
+        assert _rel(lam[:, s0:s0 + n], lo) <= 1e-4, k
+
+
+def test_bm_batched_sets_and_launch_shapes(oracle):
+    import torch
+    from cochlearmodel_amd import solver
+    _say("case: batched BM, 6 blocks over 3 signals, 2 gamma sets, waves/block 1,2,4")
+    rng = np.random.default_rng(12)
+    lens = [900, 1500, 600]
+    starts = np.cumsum([0] + lens[:-1])
+    rows = sum(lens)
+    inp = np.zeros(rows + 2, np.float32)
+    for s0, n in zip(starts, lens):
+        t = np.arange(n) / FS
+        inp[s0:s0 + n] = 10 * 2e-5 * 10 ** (rng.uniform(30, 80) / 20) * np.sin(2 * np.pi * rng.uniform(500, 4000) * t)
+    items = np.zeros(3, solver.ITEM_DTYPE)
+    for k, (s0, n) in enumerate(zip(starts, lens)):
+        items[k] = (s0, s0, s0 + n, s0, 1e-10, 1e-15, k % 2, 0)
+    gam = np.stack([np.full(256, 0.5, np.float32), np.linspace(0, 0.5, 256).astype(np.float32)])
+    sc = solver.default_scalars(FS)
+    bb = solver.BatchedBM(gam)
+    d_in = torch.from_numpy(inp).cuda()
+    outs = [bb.run(sc, d_in, items, rows, waves_per_block=w, diagnostics=True) for w in (1, 2, 4)]
+    v = [o["speeds"].cpu().numpy() for o in outs]
+    assert np.array_equal(v[0], v[1]) and np.array_equal(v[0], v[2])
+    it_o = np.zeros(3, oracle.ITEM_DTYPE)
+    for f in ("start", "mosi", "mesi", "write_start", "tol", "m1tol", "set"):
+        it_o[f] = items[f]
+    o = oracle.bm_run(oracle.default_scalars(FS), oracle.section_params(), gam, it_o, inp, rows, "f32", threads=8,
+                      diagnostics=True)
+    _say(f"rel L2 {_rel(v[0], o['speeds']):.3e} exact {np.mean(v[0] == o['speeds']):.6f}")
+    assert _rel(v[0], o["speeds"]) <= 1e-4
+    st = outs[0]["stats"].cpu().numpy()
+    np.testing.assert_array_equal(st[:, 0], o["steps"])
+    np.testing.assert_array_equal(outs[0]["failed_nodes"].cpu().numpy(), o["failed_nodes"])
+    np.testing.assert_array_equal(outs[0]["conv_loops"].cpu().numpy(), o["conv_loops"])
+
+
+def test_bm_persistent_launch_is_refused():
+    import torch
+    from cochlearmodel_amd import solver
+    from cochlearmodel_amd._lib import CochleaError
+    items = np.zeros(1, solver.ITEM_DTYPE)
+    items[0] = (0, 0, 10, 0, 1e-10, 1e-15, 0, 0)
+    bb = solver.BatchedBM(np.full(256, 0.5, np.float32))
+    with pytest.raises(CochleaError):
+        bb.run(solver.default_scalars(FS), torch.zeros(16, device="cuda"), items, 12, persistent=1)
+
+
+def _c4_small(noises=(1111.0, 40.0), freqs=(1000.0, 3000.0), levels=(30.0, 60.0), D=0.04, Tb=0.02, seed=7):
+    """A small JND sweep laid out as the reference's Sim_Type=4 run."""
+    from cochlearmodel_amd import stages
+    prof, refs, serial, calc, to_ref = stages.jnd_profiles(list(freqs), list(levels), noises=noises)
+    L = int(round(FS * D))
+    block = int(math.ceil(np.float32(Tb) * np.float32(FS)))
+    DF = L // block
+    assert DF * block == L
+    n_prof = len(prof)
+    TB = n_prof * DF
+    ov = 220
+    bufP1 = ov + block * (TB + 1)
+    head = int(round(FS * 0.014))
+    length = int(round(FS * (D - 0.014 - 0.002)))
+    WN = np.random.default_rng(seed).standard_normal(L).astype(np.float32)
+    e1 = int((D - 0.002) * FS)
+    return dict(prof=prof, refs=refs, serial=serial, calc=calc, to_ref=to_ref, L=L, block=block, DF=DF, TB=TB, ov=ov,
+                bufP1=bufP1, head=head, length=length, WN=WN, e1=e1, n_prof=n_prof)
+
+
+def _numpy_norms(c):
+    WN = c["WN"].astype(np.float64)
+    e1 = c["e1"]
+    norm = WN[0] * WN[0] + np.sum(WN[1:e1] ** 2)      # sigma accumulation with dc = 0
+    return 0.0, math.sqrt(norm), math.sqrt(0.5 * e1)
+
+
+def test_input_generation_parity(oracle):
+    from cochlearmodel_amd import _lib, stages
+    from cochlearmodel_amd._lib import check, ptr
+    _say("case: generateInputFromProfile vs oracle")
+    c = _c4_small()
+    L = _lib.lib()
+    check(L.InputProfilesArrayInitialization(c["n_prof"], c["L"], 0, 0, 0), "init")
+    from cochlearmodel_amd import solver
+    sec = solver.section_params()
+    g = np.full(256, 0.5, np.float32)
+    keep = [np.ascontiguousarray(x, np.float32) for x in (g, sec[0], sec[1], g, g, sec[2], sec[3], sec[4], sec[5],
+                                                          sec[6], sec[7])]
+    check(L.BMOHCKernel_Init(*[ptr(k) for k in keep], 1.0, 0.0, c["bufP1"] + 10, (c["bufP1"] + 10) * 256,
+                             0, 1, 0, 0), "init")
+    out = np.zeros(c["n_prof"] * c["L"], np.float32)
+    check(L.generateInputFromProfile(ptr(c["prof"]), ptr(c["WN"]), c["L"], None, 0, 0, 0, 0, 0.16, 0.0, 0, c["e1"],
+                                     0, c["e1"], c["n_prof"], 0, 0, c["n_prof"], c["ov"], c["L"], c["head"],
+                                     c["length"], FS, 1, ptr(out), 0, 1, 1e-15, 1e-10, 0, 0.0, 0.0, c["DF"], 0,
+                                     ptr(np.ones(1, np.float32)), 1, ptr(np.ones(1, np.float32)), 1), "generate")
+    wn_dc, wn_norm, sig_norm = _numpy_norms(c)
+    ref = oracle.generate_input(c["prof"]["dA"], c["prof"]["Wn"], c["prof"]["frequency"], c["WN"], wn_dc, wn_norm,
+                                sig_norm, c["L"], c["head"], FS)
+    _say(f"input rel L2 {_rel(out, ref):.3e} exact {np.mean(out == ref):.6f}")
+    assert _rel(out, ref) <= 1e-6
+
+
+def test_jnd_chain_end_to_end(oracle):
+    """Profiles -> input -> BM (decoupled blocks) -> IHC/lambda -> JND -> dB, all on
+    the GPU through the reference-named C-ABI, against the oracle chain."""
+    from cochlearmodel_amd import _lib, solver, stages
+    from cochlearmodel_amd._lib import check, ptr
+    _say("case: small JND sweep end to end (2 freq x 2 levels x 2 noises + refs)")
+    c = _c4_small()
+    L = _lib.lib()
+    sec = solver.section_params()
+    g = np.full(256, 0.5, np.float32)
+    keep = [np.ascontiguousarray(x, np.float32) for x in (g, sec[0], sec[1], g, g, sec[2], sec[3], sec[4], sec[5],
+                                                          sec[6], sec[7])]
+    n_in = c["bufP1"] + 2 * c["ov"]
+    check(L.BMOHCKernel_Init(*[ptr(k) for k in keep], 1.0, 0.0, n_in, n_in * 256, 0, 1, 0, 0), "init")
+    check(L.InputProfilesArrayInitialization(c["n_prof"], c["L"], 0, 0, 0), "init")
+    check(L.generateInputFromProfile(ptr(c["prof"]), ptr(c["WN"]), c["L"], None, 0, 0, 0, 0, 0.16, 0.0, 0, c["e1"],
+                                     0, c["e1"], c["n_prof"], 0, 0, c["n_prof"], c["ov"], c["L"], c["head"],
+                                     c["length"], FS, 0, None, 0, 1, 1e-15, 1e-10, 0, 0.0, 0.0, c["DF"], 0,
+                                     ptr(np.ones(1, np.float32)), 1, ptr(np.ones(1, np.float32)), 1), "generate")
+    sc = solver.default_scalars(FS)
+    rc = L.BMOHCNewKernel(None, 1, sc.w_ohc, sc.time_step, sc.time_step_out, sc.delta_x, sc.alpha_r, 1, 1, 0, sc.Ts,
+                          sc.alpha_l, sc.alpha_s, sc.Gme, sc.a0, sc.a1, sc.a2, sc.sigma_ow, sc.eta1, sc.eta2, c["TB"],
+                          c["bufP1"], c["ov"], 5000, 1, sc.dt_max, sc.dt_min, c["DF"], 1e-15, 1e-10, 0, 0, 0.0, 0.0,
+                          2e-5, 0, 0, 0, 2, 20, 10, 17, 2)
+    check(rc, "BMOHCNewKernel")
+    rows = c["n_prof"] * c["L"]
+    lam, _ = stages.run_ihc_reference(rows, c["block"], c["TB"], c["DF"])
+    nIHC = np.full(256, 30000 / 256, np.float64)
+    n_full = c["n_prof"]
+    F_RA = np.zeros(3 * n_full, np.float32)
+    FAI = np.zeros(3 * n_full, np.float32)
+    to_ref = c["to_ref"].astype(np.int32)
+    serial = c["serial"].astype(np.int32)
+    refs = c["refs"].astype(np.int32)
+    calc = c["calc"].astype(np.int32)
+    check(L.CudaCalculateJND(1, 1, 0, 3 * n_full, 2e-5, FS, 1.0, ptr(nIHC), ptr(calc), len(calc), ptr(refs),
+                             len(refs), 0, n_full, n_full, c["head"], c["ov"], c["L"], c["length"], ptr(serial),
+                             ptr(to_ref), ptr(F_RA), ptr(FAI), rows * 256, rows * 256, 0.2 * FS, 1, 0, 0, 0, 0),
+          "CudaCalculateJND")
+    ai, rate = stages.jnd_final(F_RA, FAI, n_full, n_full, refs, serial)
+    # oracle chain
+    wn_dc, wn_norm, sig_norm = _numpy_norms(c)
+    inp = np.zeros(c["bufP1"] + 2, np.float32)
+    inp[:rows] = oracle.generate_input(c["prof"]["dA"], c["prof"]["Wn"], c["prof"]["frequency"], c["WN"], wn_dc,
+                                       wn_norm, sig_norm, c["L"], c["head"], FS)
+    st, mo, me, ws = oracle.block_decomposition(c["TB"], c["bufP1"], c["ov"], c["DF"])
+    items = oracle.make_items(st, mo, me, ws, 1e-10, 1e-15)
+    o = oracle.bm_run(oracle.default_scalars(FS), oracle.section_params(), g, items, inp, c["bufP1"], "f32", threads=8)
+    b, _ = stages.load_ac_filter(FS)
+    lo = oracle.ihc_lambda(o["speeds"][:rows], c["L"], c["block"] * c["TB"], b, stages.dc_filter(FS),
+                           stages.ihc_damage(8.0), stages.aihc_table(), np.array(stages.SPONT_DEFAULT, np.float32),
+                           threads=8)
+    _say(f"lambda rel L2 {_rel(lam, lo):.3e}")
+    assert _rel(lam, lo) <= 1e-4
+    ref_idx = stages.reference_index(n_full, refs, serial, to_ref)
+    dA = c["prof"]["dA"].astype(np.float32)
+    mr, fra, fai = oracle.jnd(lo, n_full, c["L"], c["head"], c["length"], dA, ref_idx, nIHC,
+                              np.array(stages.W_DEFAULT, np.float32), FS, 0.2 * FS)
+    ai_o, rate_o = stages.jnd_final(fra.reshape(-1), fai.reshape(-1), n_full, n_full, refs, serial)
+    sel = calc
+    _say(f"JND AI dB gpu {np.round(ai[serial[sel]], 3)} oracle {np.round(ai_o[serial[sel]], 3)}")
+    assert np.max(np.abs(ai[serial[sel]] - ai_o[serial[sel]])) <= 0.1
+    assert np.max(np.abs(rate[serial[sel]] - rate_o[serial[sel]])) <= 0.1
+    assert np.all(np.isfinite(ai[serial[sel]]))

@@ -1,0 +1,322 @@
+"""Pins of the CPU oracle against what the mathematics and independent
+libraries fix (the reference ships no tests or golden vectors).  Each test
+is chosen so that a plausible mistake (dropped term, wrong sign or index,
+transposed operand) in the oracle fails it."""
+import math
+
+import numpy as np
+import pytest
+
+
+# ------------------------------------------------------------- math ---
+def test_tanh_defined_against_libm(oracle):
+    rng = np.random.default_rng(1)
+    xs = np.concatenate([rng.uniform(-12, 12, 4000), rng.uniform(-1, 1, 4000), 10 ** rng.uniform(-30, 0, 2000),
+                         [0.0, 0.55, 0.5499999, 9.99, 10.0, 20.0, -3.0]]).astype(np.float32)
+    worst = 0.0
+    for x in xs:
+        y = oracle.tanhf(float(x))
+        ref = math.tanh(float(x))
+        ulp = float(np.spacing(np.float32(abs(ref)))) if ref != 0 else 1e-45
+        worst = max(worst, abs(y - ref) / ulp)
+    assert worst <= 2.0
+    assert oracle.tanhf(0.0) == 0.0 and oracle.tanhf(-2.0) == -oracle.tanhf(2.0)
+
+
+def test_exp_reduced_against_libm(oracle):
+    rng = np.random.default_rng(2)
+    for x in rng.uniform(-20, 20, 4000).astype(np.float32):
+        ref = math.exp(float(x))
+        assert abs(oracle.expf(float(x)) - ref) <= 1.0 * float(np.spacing(np.float32(ref)))
+
+
+# ------------------------------------------------------------ model ---
+def test_model_vectors_closed_forms(oracle):
+    v = oracle.model_vectors()
+    M, R, S, Rtm, Stm, Sohc, Q = v
+    assert M[0] == np.float64(np.float32(1.286e-6))
+    assert R[0] == 0.25 and S[0] == np.float64(np.float32(1.282e4))
+    dx = 3.5 / 256
+    x = dx * np.arange(256)
+    np.testing.assert_allclose(M, 1.286e-6 * np.exp(1.5 * x), rtol=1e-7)
+    np.testing.assert_allclose(R, 0.25 * np.exp(-0.06 * x), rtol=1e-7)
+    np.testing.assert_allclose(S, 1.282e4 * np.exp(-1.5 * x), rtol=1e-7)
+    np.testing.assert_array_equal(Rtm, R)
+    w_ohc = 2 * np.pi * 1000
+    np.testing.assert_allclose(Stm, R * (S / M) / w_ohc, rtol=1e-6)
+    np.testing.assert_allclose(Sohc, 1e-3 * Stm, rtol=1e-7)
+    np.testing.assert_allclose(Q, 2 * 1.0 * 0.003 / (0.5 * M), rtol=1e-7)
+    # characteristic frequency sqrt(S/M)/2pi falls monotonically from ~16 kHz
+    cf = np.sqrt(S / M) / (2 * np.pi)
+    assert np.all(np.diff(cf) < 0) and 15000 < cf[0] < 17000
+
+
+def test_product_model_matches_oracle(oracle):
+    from cochlearmodel_amd import solver
+    np.testing.assert_array_equal(solver.model_vectors(), oracle.model_vectors())
+    np.testing.assert_array_equal(solver.section_params(), oracle.section_params())
+    a, b = solver.default_scalars(), oracle.default_scalars()
+    for f, _ in a._fields_:
+        assert getattr(a, f) == getattr(b, f), f
+    assert abs(b.time_step_out - 22e-6) < 1e-12 and b.dt_max == np.float32(1e-6)
+
+
+def _kernel_matrix(oracle):
+    """Dense kernel-convention tridiagonal (row 254 has no lower coupling)."""
+    m = oracle.tri_mid_diag()
+    A = np.diag(m)
+    for i in range(256):
+        if i + 1 < 256:
+            A[i, i + 1] = 1.0
+        if 0 < i < 254:
+            A[i, i - 1] = 1.0
+    return A
+
+
+def test_jacobi_fixed_point_is_dense_solution(oracle):
+    A = _kernel_matrix(oracle)
+    rng = np.random.default_rng(3)
+    Y = rng.standard_normal(256)
+    Y[255] = 0.0
+    p_star = np.linalg.solve(A, Y)
+    rsm = oracle.section_params()[1]
+    p1 = oracle.jacobi(Y.astype(np.float32), rsm, p_star.astype(np.float32), 1).astype(np.float64)
+    assert np.max(np.abs(p1 - p_star)) <= 1e-5 * np.max(np.abs(p_star))
+    # a matrix with row 254 coupled to 253 (the dead CTriDiag convention) is not the fixed point
+    A2 = A.copy()
+    A2[254, 253] = 1.0
+    p_bad = np.linalg.solve(A2, Y)
+    p1b = oracle.jacobi(Y.astype(np.float32), rsm, p_bad.astype(np.float32), 1).astype(np.float64)
+    assert np.max(np.abs(p1b - p_bad)) > 1e-3 * np.max(np.abs(p_bad))
+
+
+def test_jacobi_converges_towards_solution(oracle):
+    A = _kernel_matrix(oracle)
+    Y = np.sin(np.linspace(0, 3, 256))
+    Y[255] = 0
+    p_star = np.linalg.solve(A, Y)
+    rsm = oracle.section_params()[1]
+    e0 = np.linalg.norm(p_star)
+    p = oracle.jacobi(Y.astype(np.float32), rsm, np.zeros(256, np.float32), 3000).astype(np.float64)
+    assert np.linalg.norm(p - p_star) < 0.3 * e0
+
+
+# ------------------------------------------------------ decomposition ---
+def test_block_decomposition_voice(oracle):
+    st, mo, me, ws = oracle.block_decomposition(5, 220 + 882 * 6, 220, 0)
+    np.testing.assert_array_equal(st, [0, 882, 1764, 2646, 3528])
+    np.testing.assert_array_equal(mo, [220, 1102, 1984, 2866, 3748])
+    np.testing.assert_array_equal(me, [1102, 1984, 2866, 3748, 4630])
+    np.testing.assert_array_equal(ws, [219, 1102, 1984, 2866, 3748])
+
+
+def test_block_decomposition_decoupled(oracle):
+    DF, TB, I8, ov = 10, 40, 882, 220
+    st, mo, me, ws = oracle.block_decomposition(TB, ov + I8 * (TB + 1), ov, DF)
+    for b in range(TB):
+        if b % DF == 0:           # first block of a signal: zero state at the signal start, no transient discard
+            assert mo[b] == st[b] == b * I8 and ws[b] == mo[b]
+        if (b + 1) % DF == 0 and b != TB - 1:   # last block of a signal stops at the signal end
+            assert me[b] == (b + 1) * I8
+    # disjoint ownership covering every row once
+    owned = np.concatenate([np.arange(ws[b], me[b]) for b in range(TB)])
+    assert len(owned) == len(np.unique(owned))
+    assert owned.min() == 0 and owned.max() == me[-1] - 1 and len(owned) == me[-1]
+
+
+# ------------------------------------------------------------- BM ------
+def _voice_run(oracle, sig, TB=2, block=441, overlap=110, precision="f32", **kw):
+    sc = oracle.default_scalars(44100.0)
+    for k, v in kw.items():
+        setattr(sc, k, v)
+    bufP1 = overlap + block * (TB + 1)
+    inp = np.zeros(bufP1 + 2, np.float32)
+    inp[:min(len(sig), bufP1)] = sig[:bufP1]
+    st, mo, me, ws = oracle.block_decomposition(TB, bufP1, overlap, 0)
+    items = oracle.make_items(st, mo, me, ws, 1e-10, 1e-15)
+    out = oracle.bm_run(sc, oracle.section_params(), np.full(256, 0.5, np.float32), items, inp, bufP1, precision,
+                        threads=4, diagnostics=(precision == "f32"))
+    return out, (overlap, overlap + block * TB)
+
+
+def _tone(f, db, n, fs=44100.0):
+    t = np.arange(n) / fs
+    return (10 * 2e-5 * 10 ** (db / 20) * np.sin(2 * np.pi * f * t)).astype(np.float32)
+
+
+def test_zero_input_gives_exact_zero(oracle):
+    out, (a, b) = _voice_run(oracle, np.zeros(2000, np.float32))
+    assert np.all(out["speeds"] == 0)
+    assert np.all(out["failed_nodes"] == 0)
+
+
+def test_step_counts_and_clock(oracle):
+    out, (a, b) = _voice_run(oracle, _tone(1000, 60, 2000))
+    # dt <= dt_max = 1e-6 and the sample clock is 22 us: at least 22 steps per output sample
+    cl = out["conv_loops"][a:b]
+    assert cl.min() >= 22
+    # every committed step runs >= 2 passes (Euler + at least one corrector)
+    assert np.all(out["passes"] >= 2 * out["steps"])
+    # sweeps = 20 + 20 + 2*(P-2) per step
+    P = out["passes"]
+    S = out["steps"]
+    np.testing.assert_array_equal(out["sweeps"], 40 * S + 2 * (P - 2 * S))
+
+
+def test_time_shift_equivariance(oracle):
+    """Shifting the input by whole samples inside a block shifts the output."""
+    sig = _tone(2000, 50, 1500)
+    k = 37
+    out1, (a, b) = _voice_run(oracle, np.concatenate([np.zeros(200, np.float32), sig]))
+    out2, _ = _voice_run(oracle, np.concatenate([np.zeros(200 + k, np.float32), sig]))
+    v1 = out1["speeds"][300:500]
+    v2 = out2["speeds"][300 + k:500 + k]
+    # the FP32 sample clock (t -= 22e-6) is not exactly periodic, so the shift
+    # holds to rounding; a wrong row index or a one-sample slip fails it
+    rel = np.linalg.norm(v1 - v2) / np.linalg.norm(v1)
+    assert rel < 2e-3
+    v3 = out2["speeds"][300 + k + 1:500 + k + 1]
+    assert np.linalg.norm(v1 - v3) / np.linalg.norm(v1) > 20 * rel
+
+
+def test_low_level_linearity(oracle):
+    """At low SPL the OHC tanh is in its linear range: 20 dB -> 30 dB scales by sqrt(10)."""
+    o1, (a, b) = _voice_run(oracle, _tone(1000, 20, 2000), precision="f64")
+    o2, _ = _voice_run(oracle, _tone(1000, 30, 2000), precision="f64")
+    r = np.linalg.norm(o2["speeds"][a:b]) / np.linalg.norm(o1["speeds"][a:b])
+    assert abs(r / math.sqrt(10) - 1) < 0.02
+
+
+def test_tone_peak_tracks_characteristic_place(oracle):
+    v = oracle.model_vectors()
+    cf = np.sqrt(v[2] / v[0]) / (2 * np.pi)
+    peaks = []
+    for f in (1000.0, 4000.0):
+        o, (a, b) = _voice_run(oracle, _tone(f, 40, 1400), TB=2, block=600, overlap=110, precision="f64")
+        peaks.append(int(np.argmax(np.abs(o["speeds"][a:b]).max(0))))
+        place = int(np.argmin(np.abs(cf - f)))
+        assert abs(peaks[-1] - place) < 40
+    assert peaks[1] < peaks[0]   # higher frequency peaks nearer the base
+
+
+def test_fp32_against_fp64_rebuild(oracle):
+    o32, (a, b) = _voice_run(oracle, _tone(1000, 60, 2000))
+    o64, _ = _voice_run(oracle, _tone(1000, 60, 2000), precision="f64")
+    rel = np.linalg.norm(o32["speeds"][a:b] - o64["speeds"][a:b]) / np.linalg.norm(o64["speeds"][a:b])
+    # the fixed 20/2-sweep FP32 method differs from its FP64 rebuild at the percent level
+    assert rel < 0.05
+    assert abs(int(o32["steps"].sum()) - int(o64["steps"].sum())) <= 0.01 * o64["steps"].sum()
+
+
+def test_failure_flags_forced(oracle):
+    """Cuda_Outern_Loops=2 with a tiny tolerance forces the failure path."""
+    out, (a, b) = _voice_run(oracle, _tone(1000, 80, 1200), max_passes=2)
+    assert out["failed_nodes"][a:b].sum() > 0
+    assert np.all(out["passes"] <= 2 * out["steps"])
+
+
+def test_zero_lane_decision_mode(oracle):
+    """decision_mode=1 (what a zero-partial lane decides): always 2 passes per step."""
+    out, (a, b) = _voice_run(oracle, _tone(1000, 60, 1200), decision_mode=1)
+    np.testing.assert_array_equal(out["passes"], 2 * out["steps"])
+
+
+# --------------------------------------------------------- IHC / lambda --
+def _ihc_inputs(rows=700, seed=4):
+    rng = np.random.default_rng(seed)
+    bm = (1e-3 * rng.standard_normal((rows, 256))).astype(np.float32)
+    from cochlearmodel_amd import stages
+    b, _ = stages.load_ac_filter(44100.0)
+    dc = stages.dc_filter(44100.0)
+    return bm, b, dc, stages
+
+
+def test_ac_fir_matches_lfilter(oracle):
+    from scipy.signal import lfilter
+    bm, b, dc, stages = _ihc_inputs()
+    rows = bm.shape[0]
+    lam, st = oracle.ihc_lambda(bm, rows, rows + 1, b, dc, stages.ihc_damage(8.0), stages.aihc_table(),
+                                np.array(stages.SPONT_DEFAULT, np.float32), stages=True)
+    ac_ref = lfilter(b, [1.0], bm.astype(np.float64), axis=0)
+    # interior rows use the full window; the last row of the interval gets 0 (Q17)
+    np.testing.assert_allclose(st[0][:rows - 1], ac_ref[:rows - 1], rtol=1e-5, atol=1e-9)
+    assert np.all(st[0][rows - 1] == 0)
+    ds_ref = (bm.astype(np.float64) - st[0]) ** 2
+    np.testing.assert_allclose(st[1], ds_ref, rtol=1e-5, atol=1e-14)
+    dc_ref = lfilter(dc, [1.0], st[1].astype(np.float64), axis=0)
+    np.testing.assert_allclose(st[2][:rows - 1], dc_ref[:rows - 1], rtol=1e-5, atol=1e-14)
+
+
+def test_ac_iir_matches_lfilter(oracle):
+    from scipy.signal import butter, lfilter
+    bm, _, dc, stages = _ihc_inputs(rows=500)
+    b, a = butter(4, 1000 / 22050)
+    rows = bm.shape[0]
+    lam, st = oracle.ihc_lambda(bm, 0, rows + 1, b, dc, stages.ihc_damage(8.0), stages.aihc_table(),
+                                np.array(stages.SPONT_DEFAULT, np.float32), stages=True, ac_a=a)
+    # the reference IIR truncates both histories to the taps used so far; with
+    # a length-5 filter that equals lfilter from rest except at the last row.
+    ac_ref = lfilter(b, a, bm.astype(np.float64), axis=0)
+    # FP32 accumulation in the recursion (the <float,float> instance): errors scale with the signal
+    err = np.abs(st[0][:rows - 1] - ac_ref[:rows - 1])
+    assert err.max() <= 1e-3 * np.abs(ac_ref).max()   # direct-form poles amplify FP32 rounding ~1e3x
+    assert np.all(st[0][rows - 1] == 0)
+
+
+def test_lambda_closed_form(oracle):
+    bm, b, dc, stages = _ihc_inputs(rows=400)
+    rows = bm.shape[0]
+    lam, st = oracle.ihc_lambda(bm, rows, rows + 1, b, dc, stages.ihc_damage(8.0), stages.aihc_table(),
+                                np.array(stages.SPONT_DEFAULT, np.float32), stages=True)
+    ac, ds, dcv, pre, ihc = (x.astype(np.float64) for x in st)
+    # one FP32 rounding of eta_AC*AC + eta_DC*DC (the terms may cancel)
+    assert np.all(np.abs(pre - (ac + 100.0 * dcv)) <= 1.2e-7 * (np.abs(ac) + 100.0 * np.abs(dcv)) + 1e-30)
+    np.testing.assert_allclose(ihc, np.log10(np.maximum(1e8 * pre, 0) + 1e-20), rtol=1e-6, atol=1e-6)
+    for j, (A, sp) in enumerate(zip(stages.AIHC_DEFAULT, stages.SPONT_DEFAULT)):
+        exp = np.minimum(500.0, np.maximum(A * ihc, sp))
+        exp[:, 0] = sp
+        np.testing.assert_allclose(lam[j], exp, rtol=1e-6)
+
+
+# --------------------------------------------------------------- JND -----
+def test_jnd_analytic(oracle):
+    """Constant reference rate and a linear lambda difference give closed-form Fisher sums."""
+    n_int, L, head, ln = 2, 50, 5, 40
+    rows = n_int * L
+    lam = np.zeros((3, rows, 256), np.float32)
+    base = np.array([60.0, 3.0, 0.1], np.float32)
+    d = np.float32(0.5)
+    for j in range(3):
+        lam[j, 0:L] = base[j] + d      # interval 0: signal
+        lam[j, L:2 * L] = base[j]      # interval 1: reference
+    dA = np.array([2.0, 0.0], np.float32)
+    ref = np.array([1, 1], np.int32)
+    nIHC = np.full(256, 30000 / 256, np.float64)
+    W = np.array([0.61, 0.23, 0.16], np.float32)
+    Fs = 44100.0
+    T = 0.2 * Fs
+    mr, fra, fai = oracle.jnd(lam, n_int, L, head, ln, dA, ref, nIHC, W, Fs, T)
+    Tf = T / Fs
+    Tl = ln / Fs
+    for j in range(3):
+        dl = d / dA[0]
+        exp_ai = 256 * (Tf / Tl) * nIHC[0] * ln * dl ** 2 / (Fs * base[j]) * W[j]
+        exp_rms = 256 * nIHC[0] * Tf * dl ** 2 / base[j] * W[j]
+        assert abs(fai[j, 0] / exp_ai - 1) < 1e-4
+        assert abs(fra[j, 0] / exp_rms - 1) < 1e-4
+    from cochlearmodel_amd import stages
+    ai, rate = stages.jnd_final(fra, fai, 2, 2, np.array([1], np.int32), np.array([0, 0], np.int32))
+    exp = 20 * np.log10(1 / math.sqrt(float(fai[:, 0].astype(np.float64).sum())) / 2e-5)
+    assert abs(ai[0] - exp) < 1e-9
+
+
+def test_input_generation_closed_form(oracle):
+    rng = np.random.default_rng(5)
+    L, head, Fs = 300, 20, 44100.0
+    WN = rng.standard_normal(L).astype(np.float32)
+    dA = np.array([0.01, 0.2]); Wn = np.array([0.0, 0.05]); fr = np.array([500.0, 1500.0])
+    out = oracle.generate_input(dA, Wn, fr, WN, 0.0, 3.0, 7.0, L, head, Fs)
+    n = np.arange(L)
+    for p in range(2):
+        exp = dA[p] * np.cos(2 * np.float32(np.pi) * fr[p] * (n - head) / Fs) / 7.0 + Wn[p] * WN / 3.0
+        np.testing.assert_allclose(out[p * L:(p + 1) * L], exp, rtol=1e-6, atol=1e-9)
