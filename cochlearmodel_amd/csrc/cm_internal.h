@@ -64,8 +64,8 @@ int bm_ensure_input(size_t n);
 extern "C" int cm_launch_bm(const cm::BmLaunch *a, int waves_per_block, int grid_blocks, hipStream_t stream);
 extern "C" int cm_launch_ihc_lambda(const float *d_bm, int64_t rows, const cm_ihc_interval *d_ivs,
                                     const cm_ihc_interval *h_ivs, int n_ivs, const cm_ihc_interval *d_iir_ivs,
-                                    int n_iir_ivs, const double *d_ac_b, const double *d_ac_a, int n_ac,
-                                    const double *d_dc, int n_dc, const double *d_damage, const float *d_A,
+                                    int n_iir_ivs, const double *d_ac_b, const double *d_ac_bpad,
+                                    const double *d_ac_a, int n_ac, const double *d_dc, const double *d_dcpad, int n_dc, const double *d_damage, const float *d_A,
                                     const float *spont3, float lambda_sat, float eta_ac, float eta_dc, float scale_bm,
                                     float *d_scratch, float *d_lambda, float *d_stages, hipStream_t stream);
 extern "C" int cm_launch_jnd(const cm::JndArgs *a, hipStream_t stream);

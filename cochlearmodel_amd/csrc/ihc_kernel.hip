@@ -27,31 +27,117 @@ __device__ __forceinline__ int fir_window(int64_t k, const cm_ihc_interval &iv, 
   return (int)(w < taps ? w : taps);
 }
 
+constexpr int kRows = 16;   // outputs per thread (register block) = taps per chunk
+
+// General register-blocked FIR over the n <= kRows rows row_first.. of one
+// section: inputs stream from the newest output row down, so each output
+// accumulates fma(b[0],x[k]), fma(b[1],x[k-1]), ... in tap order; a tap
+// outside an output's window contributes an exact zero.  No row past the
+// block or before row 0 is read.
+__device__ __forceinline__ void fir_block_general(const float *__restrict__ src, int64_t row_first, int n, int s,
+                                                  const double *__restrict__ b, int taps, const int (&w)[kRows],
+                                                  double (&acc)[kRows]) {
+#pragma unroll
+  for (int i = 0; i < kRows; ++i) acc[i] = 0.0;
+  const int top = n - 1;
+  const int span = n + taps - 1;
+  for (int jj = 0; jj < span; ++jj) {
+    const int64_t row = row_first + top - jj;
+    if (row < 0) break;
+    const double x = (double)src[row * CM_SECTIONS + s];
+#pragma unroll
+    for (int i = 0; i < kRows; ++i) {
+      const int t = jj - top + i;
+      const bool in = t >= 0 && t < w[i];
+      const double c = in ? b[t] : 0.0;
+      acc[i] = __builtin_fma(c, x, acc[i]);
+    }
+  }
+}
+
+// Steady-state FIR (every output uses all taps): taps in chunks of kRows with
+// a 2*kRows-1 row register window; coefficients are uniform per tap and the
+// tap array is zero-padded to a whole number of chunks.  Output i takes tap
+// t from row row_first + i - t, accumulated in ascending t.  Loads are
+// clamped to [0, row_first + n - 1]; clamped rows only meet zero taps or
+// outputs past n.
+__device__ __forceinline__ void fir_block_steady(const float *__restrict__ src, int64_t row_first, int n, int s,
+                                                 const double *__restrict__ bpad, int chunks, double (&acc)[kRows]) {
+#pragma unroll
+  for (int i = 0; i < kRows; ++i) acc[i] = 0.0;
+  const int64_t hi = row_first + n - 1;
+  double X[2 * kRows - 1];   // X[m] = row row_first - 16c - 15 + m
+#pragma unroll
+  for (int m = kRows - 1; m < 2 * kRows - 1; ++m) {
+    int64_t r = row_first - (kRows - 1) + m;
+    r = r > hi ? hi : r;
+    X[m] = (double)src[r * CM_SECTIONS + s];
+  }
+  for (int c = 0; c < chunks; ++c) {
+#pragma unroll
+    for (int m = 0; m < kRows; ++m) {
+      int64_t r = row_first - (int64_t)kRows * c - (kRows - 1) + m;
+      r = r < 0 ? 0 : r;
+      X[m] = (double)src[r * CM_SECTIONS + s];
+    }
+    const double *bc = bpad + kRows * c;
+#pragma unroll
+    for (int u = 0; u < kRows; ++u) {
+      const double coef = bc[u];
+#pragma unroll
+      for (int i = 0; i < kRows; ++i) acc[i] = __builtin_fma(coef, X[i - u + kRows - 1], acc[i]);
+    }
+#pragma unroll
+    for (int m = 2 * kRows - 2; m >= kRows; --m) X[m] = X[m - kRows];
+  }
+}
+
+// Window of kRows outputs starting at interval row r0; returns 1 when every
+// output of the block uses all taps.
+__device__ __forceinline__ int block_windows(const cm_ihc_interval &iv, int64_t r0, int n, int taps, int (&w)[kRows]) {
+  int steady = 1;
+#pragma unroll
+  for (int i = 0; i < kRows; ++i) {
+    w[i] = i < n ? fir_window(iv.kbase + r0 + i, iv, taps) : 0;
+    if (i < n && w[i] != taps) steady = 0;
+  }
+  return steady;
+}
+
 // AC (FIR, or read back when an IIR pass already wrote it) and dS.
 __global__ __launch_bounds__(256) void ac_ds_kernel(const float *__restrict__ bm, const cm_ihc_interval *__restrict__ ivs,
-                                                    const double *__restrict__ b, int nb, float coefA, double coefB,
-                                                    int rows_per_block, int ac_ready, float *__restrict__ AC,
-                                                    float *__restrict__ dS) {
+                                                    const double *__restrict__ b, const double *__restrict__ bpad,
+                                                    int nb, float coefA, double coefB, int ac_ready,
+                                                    float *__restrict__ AC, float *__restrict__ dS) {
   const cm_ihc_interval iv = ivs[blockIdx.y];
   const int s = threadIdx.x;
-  const int64_t r0 = (int64_t)blockIdx.x * rows_per_block;
+  const int64_t r0 = (int64_t)blockIdx.x * kRows;
   if (r0 >= iv.len) return;
-  const int64_t r1 = r0 + rows_per_block < iv.len ? r0 + rows_per_block : iv.len;
-  for (int64_t r = r0; r < r1; ++r) {
-    const int64_t row = iv.row0 + r;
-    const int64_t k = iv.kbase + r;
-    float ac;
-    if (ac_ready) {
-      ac = AC[row * CM_SECTIONS + s];
-    } else {
-      const int w = fir_window(k, iv, nb);
-      double y = 0.0;
-      for (int i = 0; i < w; ++i) y = y + b[i] * (double)bm[(row - i) * CM_SECTIONS + s];
-      ac = (float)y;
-      AC[row * CM_SECTIONS + s] = ac;
+  const int n = iv.len - r0 < kRows ? (int)(iv.len - r0) : kRows;
+  float ac[kRows];
+  if (ac_ready) {
+#pragma unroll
+    for (int i = 0; i < kRows; ++i) ac[i] = i < n ? AC[(iv.row0 + r0 + i) * CM_SECTIONS + s] : 0.0f;
+  } else {
+    int w[kRows];
+    double acc[kRows];
+    if (block_windows(iv, r0, n, nb, w))
+      fir_block_steady(bm, iv.row0 + r0, n, s, bpad, (nb + kRows - 1) / kRows, acc);
+    else
+      fir_block_general(bm, iv.row0 + r0, n, s, b, nb, w, acc);
+#pragma unroll
+    for (int i = 0; i < kRows; ++i) {
+      ac[i] = (float)acc[i];
+      if (i < n) AC[(iv.row0 + r0 + i) * CM_SECTIONS + s] = ac[i];
     }
-    const float mid = __builtin_fmaf(bm[row * CM_SECTIONS + s], coefA, (float)((double)ac * coefB));
-    dS[row * CM_SECTIONS + s] = mid * mid;
+  }
+#pragma unroll
+  for (int i = 0; i < kRows; ++i) {
+    if (i < n) {
+      const int64_t o = (iv.row0 + r0 + i) * CM_SECTIONS + s;
+      const float mid = __builtin_fmaf(bm[o], coefA, (float)((double)ac[i] * coefB));
+      dS[o] = mid * mid;
+    }
   }
 }
 
@@ -78,6 +164,7 @@ struct LambdaArgs {
   const float *dS;
   const cm_ihc_interval *ivs;
   const double *dc;
+  const double *dcpad;    // dc zero-padded to a multiple of kRows taps
   int ndc;
   float etaAC;
   double etaDC;
@@ -88,16 +175,15 @@ struct LambdaArgs {
   int64_t rows;           // lambda plane stride in rows
   float *lambda;          // 3 x rows x 256
   float *stages;          // optional 5 x rows x 256 (AC, dS, DC, PRE_IHC, IHC)
-  int rows_per_block;
 };
 
 // DC box-car FIR, PRE_IHC, IHC and the three lambdas.
 __global__ __launch_bounds__(256) void dc_lambda_kernel(LambdaArgs a) {
   const cm_ihc_interval iv = a.ivs[blockIdx.y];
   const int s = threadIdx.x;
-  const int64_t r0 = (int64_t)blockIdx.x * a.rows_per_block;
+  const int64_t r0 = (int64_t)blockIdx.x * kRows;
   if (r0 >= iv.len) return;
-  const int64_t r1 = r0 + a.rows_per_block < iv.len ? r0 + a.rows_per_block : iv.len;
+  const int n = iv.len - r0 < kRows ? (int)(iv.len - r0) : kRows;
   const double dmg = a.damage[(int64_t)iv.set * CM_SECTIONS + s];
   const double zero_factor = s != 0 ? 1.0 : 0.0;
   double A3[3], sp[3], zo[3];
@@ -106,15 +192,18 @@ __global__ __launch_bounds__(256) void dc_lambda_kernel(LambdaArgs a) {
     sp[j] = (double)a.spont[j];
     zo[j] = s != 0 ? 0.0 : sp[j];
   }
+  int w[kRows];
+  double acc[kRows];
+  if (block_windows(iv, r0, n, a.ndc, w))
+    fir_block_steady(a.dS, iv.row0 + r0, n, s, a.dcpad, (a.ndc + kRows - 1) / kRows, acc);
+  else
+    fir_block_general(a.dS, iv.row0 + r0, n, s, a.dc, a.ndc, w, acc);
   const int64_t plane = a.rows * CM_SECTIONS;
-  for (int64_t r = r0; r < r1; ++r) {
-    const int64_t row = iv.row0 + r;
-    const int64_t k = iv.kbase + r;
-    const int w = fir_window(k, iv, a.ndc);
-    double y = 0.0;
-    for (int i = 0; i < w; ++i) y = y + a.dc[i] * (double)a.dS[(row - i) * CM_SECTIONS + s];
-    const float dc = (float)y;
-    const int64_t o = row * CM_SECTIONS + s;
+#pragma unroll
+  for (int i = 0; i < kRows; ++i) {
+    if (i >= n) continue;
+    const int64_t o = (iv.row0 + r0 + i) * CM_SECTIONS + s;
+    const float dc = (float)acc[i];
     const float ac = a.AC[o];
     const float pre = __builtin_fmaf(ac, a.etaAC, (float)((double)dc * a.etaDC));
     const float ihc = (float)log10(fmax((double)(dmg * (double)pre), 0.0) + 1e-20);
@@ -138,16 +227,15 @@ __global__ __launch_bounds__(256) void dc_lambda_kernel(LambdaArgs a) {
 // 2 x rows x 256 floats (AC, dS).
 extern "C" int cm_launch_ihc_lambda(const float *d_bm, int64_t rows, const cm_ihc_interval *d_ivs,
                                     const cm_ihc_interval *h_ivs, int n_ivs, const cm_ihc_interval *d_iir_ivs,
-                                    int n_iir_ivs, const double *d_ac_b, const double *d_ac_a, int n_ac,
-                                    const double *d_dc, int n_dc, const double *d_damage, const float *d_A,
+                                    int n_iir_ivs, const double *d_ac_b, const double *d_ac_bpad,
+                                    const double *d_ac_a, int n_ac, const double *d_dc, const double *d_dcpad, int n_dc, const double *d_damage, const float *d_A,
                                     const float *spont3, float lambda_sat, float eta_ac, float eta_dc, float scale_bm,
                                     float *d_scratch, float *d_lambda, float *d_stages, hipStream_t stream) {
   using namespace cm;
   if (n_ivs <= 0) return 0;
   int64_t max_len = 0;
   for (int i = 0; i < n_ivs; ++i) max_len = h_ivs[i].len > max_len ? h_ivs[i].len : max_len;
-  const int rpb = 32;
-  const int64_t gx = (max_len + rpb - 1) / rpb;
+  const int64_t gx = (max_len + kRows - 1) / kRows;
   if (gx > 0x7fffffff || n_ivs > 65535) return -3;
   float *AC = d_scratch;
   float *dS = d_scratch + rows * CM_SECTIONS;
@@ -156,13 +244,14 @@ extern "C" int cm_launch_ihc_lambda(const float *d_bm, int64_t rows, const cm_ih
     hipLaunchKernelGGL(ac_iir_kernel, dim3(n_iir_ivs), dim3(256), 0, stream, d_bm, d_iir_ivs, d_ac_b, d_ac_a, n_ac,
                        AC);
   }
-  hipLaunchKernelGGL(ac_ds_kernel, dim3((unsigned)gx, n_ivs), dim3(256), 0, stream, d_bm, d_ivs, d_ac_b, n_ac,
-                     scale_bm, -1.0, rpb, iir, AC, dS);
+  hipLaunchKernelGGL(ac_ds_kernel, dim3((unsigned)gx, n_ivs), dim3(256), 0, stream, d_bm, d_ivs, d_ac_b, d_ac_bpad,
+                     n_ac, scale_bm, -1.0, iir, AC, dS);
   LambdaArgs a;
   a.AC = AC;
   a.dS = dS;
   a.ivs = d_ivs;
   a.dc = d_dc;
+  a.dcpad = d_dcpad;
   a.ndc = n_dc;
   a.etaAC = eta_ac;
   a.etaDC = (double)eta_dc;
@@ -173,7 +262,6 @@ extern "C" int cm_launch_ihc_lambda(const float *d_bm, int64_t rows, const cm_ih
   a.rows = rows;
   a.lambda = d_lambda;
   a.stages = d_stages;
-  a.rows_per_block = rpb;
   hipLaunchKernelGGL(dc_lambda_kernel, dim3((unsigned)gx, n_ivs), dim3(256), 0, stream, a);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }

@@ -46,7 +46,7 @@ template <typename T> struct Buf {
 // cudaLambdaHolderData / cudaJNDHolder / cudaSignalHolder / cudaModelAihc of
 // src/cochlea.cu:63-631.
 struct StageHolder {
-  Buf<double> ac_b, ac_a, dc, damage, nIHC;
+  Buf<double> ac_b, ac_a, dc, damage, nIHC, ac_bpad, dcpad;
   Buf<float> A, lambda, scratch, buffer1, stages;
   Buf<cm_ihc_interval> ivs, iir_ivs;
   Buf<cm_jnd_profile> profiles;
@@ -155,6 +155,15 @@ int releaseBuffer(int Show_Run_Time) {
   return 0;
 }
 
+// Zero-padded copy of a device tap array (a whole number of 16-tap chunks).
+static int pad_taps(Buf<double> &dst, const double *d_src, int n) {
+  const size_t np = (size_t)((n + 15) / 16) * 16;
+  if (dst.ensure(np)) return -1;
+  if (check(hipMemsetAsync(dst.ptr, 0, np * sizeof(double), stream()), "pad taps")) return -1;
+  return check(hipMemcpyAsync(dst.ptr, d_src, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, stream()),
+               "pad taps");
+}
+
 static void build_intervals(int rows, std::vector<cm_ihc_interval> &fir, std::vector<cm_ihc_interval> &iir) {
   const int DF = g_st.decouple;
   const int L = DF > 0 ? g_st.interval_nodes * DF : rows;
@@ -194,9 +203,10 @@ int RunIHCKernel(float *host_backup, int Show_Run_Time, int save_lambda, int bac
   const int want_stages = backup_stage >= 9 && backup_stage <= 14 && host_backup;
   if (want_stages && g_st.stages.ensure(5 * plane)) return -1;
   if (g_st.lambda.ensure(3 * plane) || g_st.scratch.ensure(2 * plane)) return -1;
+  if (pad_taps(g_st.ac_bpad, g_st.ac_b.ptr, g_st.n_ac) || pad_taps(g_st.dcpad, g_st.dc.ptr, g_st.n_dc)) return -1;
   int rc = cm_launch_ihc_lambda(bm_speeds_ptr(), rows, g_st.ivs.ptr, fir.data(), (int)fir.size(), g_st.iir_ivs.ptr,
-                                (int)iir.size(), g_st.ac_b.ptr, g_st.is_iir ? g_st.ac_a.ptr : nullptr, g_st.n_ac,
-                                g_st.dc.ptr, g_st.n_dc, g_st.damage.ptr, g_st.A.ptr, g_st.spont, g_st.sat,
+                                (int)iir.size(), g_st.ac_b.ptr, g_st.ac_bpad.ptr, g_st.is_iir ? g_st.ac_a.ptr : nullptr,
+                                g_st.n_ac, g_st.dc.ptr, g_st.dcpad.ptr, g_st.n_dc, g_st.damage.ptr, g_st.A.ptr, g_st.spont, g_st.sat,
                                 g_st.eta_ac, g_st.eta_dc, g_st.scale, g_st.scratch.ptr, g_st.lambda.ptr,
                                 want_stages ? g_st.stages.ptr : nullptr, stream());
   if (rc) {
@@ -257,8 +267,9 @@ int cm_ihc_lambda_run(const float *d_bm, int64_t rows, const cm_ihc_interval *h_
     return -3;
   }
   if (d_ac_a && g_st.iir_ivs.upload(h_iir_ivs, n_iir_ivs)) return -1;
+  if (pad_taps(g_st.ac_bpad, d_ac_b, n_ac) || pad_taps(g_st.dcpad, d_dc, n_dc)) return -1;
   const int rc = cm_launch_ihc_lambda(d_bm, rows, g_st.ivs.ptr, h_ivs, n_ivs, g_st.iir_ivs.ptr, n_iir_ivs, d_ac_b,
-                                      d_ac_a, n_ac, d_dc, n_dc, d_damage, d_A, spont3, lambda_sat, eta_ac, eta_dc,
+                                      g_st.ac_bpad.ptr, d_ac_a, n_ac, d_dc, g_st.dcpad.ptr, n_dc, d_damage, d_A, spont3, lambda_sat, eta_ac, eta_dc,
                                       scale_bm, d_scratch, d_lambda, d_stages, stream());
   if (rc) set_error("cm_ihc_lambda_run: launch failed (%d)", rc);
   return rc ? -1 : 0;
@@ -631,6 +642,8 @@ int cm_generate_input_run(const cm_jnd_profile *d_profiles, const float *d_WN, c
 int IHCKernel_Free(void) {
   g_st.ac_b.release();
   g_st.ac_a.release();
+  g_st.ac_bpad.release();
+  g_st.dcpad.release();
   g_st.dc.release();
   g_st.damage.release();
   g_st.nIHC.release();

@@ -468,7 +468,9 @@ int orc_jacobi(const float *Y, const float *rsm, float *p, int sweeps) {
 // ------------------------------------------------------------ IHC ------
 // FIR window of DeviceCudaFIRFilter (src/cochlea.cu:2712-2728): taps used at
 // node k are min((k+1) mod period, taps), period = interval length before the
-// final regular division and the whole analysed length after it (Q17).
+// final regular division and the whole analysed length after it (Q17).  The
+// reference's Y = Y + c*x accumulation is taken as a fused multiply-add in
+// tap order (what nvcc's default contraction makes of it).
 static inline int fir_window(int64_t k, int64_t tla, int64_t L, int64_t frdp, int taps) {
   int64_t sg = frdp > 1 ? k / (frdp - 1) : 1;
   int64_t per = sg * tla + (1 - sg) * L;
@@ -524,7 +526,7 @@ int orc_ihc_lambda(const float *bm, int64_t rows, int interval_len, int frdp, co
       for (int64_t k = 0; k < rows; ++k) { // AC FIR, Eq. 5.1 (double accumulate)
         int w = fir_window(k, tla, L, frdp, n_ac);
         double y = 0.0;
-        for (int i = 0; i < w; ++i) y = y + ac_b[i] * (double)bm[(k - i) * NS + s];
+        for (int i = 0; i < w; ++i) y = std::fma(ac_b[i], (double)bm[(k - i) * NS + s], y);
         AC[k] = (float)y;
       }
     }
@@ -535,7 +537,7 @@ int orc_ihc_lambda(const float *bm, int64_t rows, int interval_len, int frdp, co
     for (int64_t k = 0; k < rows; ++k) {   // DC box-car FIR, Eq. 5.3
       int w = fir_window(k, tla, L, frdp, n_dc);
       double y = 0.0;
-      for (int i = 0; i < w; ++i) y = y + dc_b[i] * (double)dS[k - i];
+      for (int i = 0; i < w; ++i) y = std::fma(dc_b[i], (double)dS[k - i], y);
       DC[k] = (float)y;
     }
     const double dmg = ihc_damage[s];
