@@ -23,7 +23,8 @@ class BMScalars(C.Structure):
         "w_ohc", "time_step", "time_step_out", "delta_x", "alpha_r", "Ts", "alpha_l", "alpha_s",
         "Gme", "a0", "a1", "a2", "sigma_ow", "eta1", "eta2", "dt_min", "dt_max")] + [
         ("j_fast", C.c_int32), ("j_slow", C.c_int32), ("max_passes", C.c_int32),
-        ("decision_mode", C.c_int32)]
+        ("decision_mode", C.c_int32), ("jacobi_mode", C.c_int32), ("check_every", C.c_int32),
+        ("max_sweeps", C.c_int32), ("tol_rel", C.c_float)]
 
 
 ITEM_DTYPE = np.dtype([("start", np.int32), ("mosi", np.int32), ("mesi", np.int32),
@@ -115,6 +116,13 @@ _SIGS = {
     "cm_tri_matrix": (C.c_int, [vp, vp, vp]),
     "cm_pack_section_params": (C.c_int, [vp]),
     "cm_default_bm_scalars": (C.c_int, [f32, vp]),
+    "cm_gemm_bf16": (C.c_int, [C.c_int] * 5 + [vp, C.c_int, vp, C.c_int, vp, C.c_int, vp, f32]),
+    "cm_gelu_fwd": (C.c_int, [vp, vp, vp, i64]),
+    "cm_gelu_bwd": (C.c_int, [vp, vp, vp, vp, i64]),
+    "cm_to_bf16": (C.c_int, [vp, vp, i64]),
+    "cm_colsum": (C.c_int, [vp, vp, C.c_int, C.c_int]),
+    "cm_mse_grad": (C.c_int, [vp, vp, vp, vp, vp, C.c_int, C.c_int]),
+    "cm_adam": (C.c_int, [vp, vp, vp, vp, vp, i64, f32, f32, f32, f32, C.c_int, f32]),
 }
 
 

@@ -103,13 +103,31 @@ __device__ __forceinline__ float in_at(const BmLaunch &a, int64_t idx) {
   return (idx >= 0 && idx < a.input_len) ? a.input[idx] : 0.0f;
 }
 
-// One time block, include/cochlea.cuh:13620-14232.
+// J Jacobi sweeps of the kernel convention (include/cochlea.cuh:13962-13982):
+// p_i <- (Y_i - (l_i p_{i-1} + p_{i+1})) / m_i with the lane-boundary
+// neighbours exchanged by DPP wave shifts (zero past either end).
+__device__ __forceinline__ void jacobi_sweeps(const float (&Y)[4], const float (&rsm)[4], float low23, float (&p)[4],
+                                              int J) {
+  for (int k = 0; k < J; ++k) {
+    const float left = wave_shr1(p[3]);
+    const float right = wave_shl1(p[0]);
+    const float q0 = (Y[0] - (left + p[1])) * rsm[0];
+    const float q1 = (Y[1] - (p[0] + p[2])) * rsm[1];
+    const float q2 = (Y[2] - (low23 * p[1] + p[3])) * rsm[2];
+    const float q3 = (Y[3] - (low23 * p[2] + right)) * rsm[3];
+    p[0] = q0; p[1] = q1; p[2] = q2; p[3] = q3;
+  }
+}
+
+// One time block, include/cochlea.cuh:13620-14232.  TOL selects the
+// sweep-to-tolerance Jacobi; otherwise the reference's fixed 20/2 sweeps.
+template <bool TOL>
 __device__ void run_item(const BmLaunch &a, int item_index, int lane) {
   const cm_bm_item it = a.items[item_index];
   const cm_bm_scalars &sc = a.sc;
   const int base = 4 * lane;
 
-  float rsm[4], rmass[4], gam[4], R[4], S[4], Q[4], Sohc[4], Stm[4], Rtm[4];
+  float rsm[4], rmass[4], gam[4], R[4], S[4], Q[4], Sohc[4], Stm[4], Rtm[4], mf[4];
   {
     const float4 m4 = *reinterpret_cast<const float4 *>(a.sec + CM_SEC_MASS * CM_SECTIONS + base);
     const float4 r4 = *reinterpret_cast<const float4 *>(a.sec + CM_SEC_RSM * CM_SECTIONS + base);
@@ -130,6 +148,12 @@ __device__ void run_item(const BmLaunch &a, int item_index, int lane) {
     Rtm[0] = U4.x; Rtm[1] = U4.y; Rtm[2] = U4.z; Rtm[3] = U4.w;
     gam[0] = G4.x; gam[1] = G4.y; gam[2] = G4.z; gam[3] = G4.w;
     UNROLL for (int j = 0; j < 4; ++j) rmass[j] = 1.0f / mm[j];
+    if (TOL) {
+      const float4 M4 = *reinterpret_cast<const float4 *>(a.sec + CM_SEC_M * CM_SECTIONS + base);
+      mf[0] = M4.x; mf[1] = M4.y; mf[2] = M4.z; mf[3] = M4.w;
+    } else {
+      UNROLL for (int j = 0; j < 4; ++j) mf[j] = 0.0f;
+    }
   }
   // low multiplier: 0 for sections 254 and 255 (lane 63, slots 2 and 3)
   const float low23 = lane == 63 ? 0.0f : 1.0f;
@@ -217,15 +241,28 @@ __device__ void run_item(const BmLaunch &a, int item_index, int lane) {
       if (lane == 0) Y[0] = Y0;
       if (lane == 63) Y[3] = 0.0f;
 
-      const int J = aux_pass ? sc.j_slow : sc.j_fast;
-      for (int k = 0; k < J; ++k) {
-        const float left = wave_shr1(p[3]);
-        const float right = wave_shl1(p[0]);
-        const float q0 = (Y[0] - (left + p[1])) * rsm[0];
-        const float q1 = (Y[1] - (p[0] + p[2])) * rsm[1];
-        const float q2 = (Y[2] - (low23 * p[1] + p[3])) * rsm[2];
-        const float q3 = (Y[3] - (low23 * p[2] + right)) * rsm[3];
-        p[0] = q0; p[1] = q1; p[2] = q2; p[3] = q3;
+      int J;
+      if (TOL) {
+        // sweep until |Y - A p|_2 <= tol |Y|_2, checking every check_every sweeps
+        const float yy = tree_sum(Y[0] * Y[0], Y[1] * Y[1], Y[2] * Y[2], Y[3] * Y[3]);
+        const float tau2 = sc.tol_rel * sc.tol_rel;
+        J = 0;
+        for (;;) {
+          const float left = wave_shr1(p[3]);
+          const float right = wave_shl1(p[0]);
+          const float r0 = Y[0] - ((left + p[1]) + mf[0] * p[0]);
+          const float r1 = Y[1] - ((p[0] + p[2]) + mf[1] * p[1]);
+          const float r2 = Y[2] - ((low23 * p[1] + p[3]) + mf[2] * p[2]);
+          const float r3 = Y[3] - ((low23 * p[2] + right) + mf[3] * p[3]);
+          const float rr = tree_sum(r0 * r0, r1 * r1, r2 * r2, r3 * r3);
+          if (rr <= tau2 * yy || J >= sc.max_sweeps) break;
+          const int kk = sc.check_every < sc.max_sweeps - J ? sc.check_every : sc.max_sweeps - J;
+          jacobi_sweeps(Y, rsm, low23, p, kk);
+          J += kk;
+        }
+      } else {
+        J = aux_pass ? sc.j_slow : sc.j_fast;
+        jacobi_sweeps(Y, rsm, low23, p, J);
       }
       n_sweeps += J;
       UNROLL for (int j = 0; j < 4; ++j) na[j] = (p[j] + G[j]) * rmass[j];
@@ -322,42 +359,29 @@ __device__ void run_item(const BmLaunch &a, int item_index, int lane) {
   }
 }
 
-template <int WAVES>
+template <int WAVES, bool TOL>
 __global__ __launch_bounds__(64 * WAVES) void bm_kernel(BmLaunch a) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  if (!a.use_queue) {
-    const int idx = blockIdx.x * WAVES + wave;
-    if (idx < a.n_items) run_item(a, idx, lane);
-    return;
-  }
-  // persistent: waves pull blocks from a counter until the table is drained
-  for (;;) {
-    int idx = 0;
-    if (lane == 0) idx = atomicAdd(a.queue, 1);
-    idx = __builtin_amdgcn_readfirstlane(idx);
-    if (idx >= a.n_items) break;
-    run_item(a, idx, lane);
-  }
+  const int idx = blockIdx.x * WAVES + wave;
+  if (idx < a.n_items) run_item<TOL>(a, idx, lane);
 }
 
-__global__ void fill_i32(int32_t *p, int32_t v, int64_t n) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = v;
-}
 
 }  // namespace cm
 
 // Launcher used by the C-ABI layer.
 extern "C" int cm_launch_bm(const cm::BmLaunch *a, int waves_per_block, int grid_blocks, hipStream_t stream) {
-  if (a->use_queue) {
-    hipLaunchKernelGGL(cm::fill_i32, dim3(1), dim3(64), 0, stream, a->queue, 0, (int64_t)1);
-  }
+  if (a->use_queue) return -1;   // the work-queue launch is switched off
   dim3 grid(grid_blocks);
-  switch (waves_per_block) {
-    case 1: hipLaunchKernelGGL(cm::bm_kernel<1>, grid, dim3(64), 0, stream, *a); break;
-    case 2: hipLaunchKernelGGL(cm::bm_kernel<2>, grid, dim3(128), 0, stream, *a); break;
-    case 4: hipLaunchKernelGGL(cm::bm_kernel<4>, grid, dim3(256), 0, stream, *a); break;
+  const bool tol = a->sc.jacobi_mode == 1;
+  switch (waves_per_block * 2 + (tol ? 1 : 0)) {
+    case 2: hipLaunchKernelGGL((cm::bm_kernel<1, false>), grid, dim3(64), 0, stream, *a); break;
+    case 3: hipLaunchKernelGGL((cm::bm_kernel<1, true>), grid, dim3(64), 0, stream, *a); break;
+    case 4: hipLaunchKernelGGL((cm::bm_kernel<2, false>), grid, dim3(128), 0, stream, *a); break;
+    case 5: hipLaunchKernelGGL((cm::bm_kernel<2, true>), grid, dim3(128), 0, stream, *a); break;
+    case 8: hipLaunchKernelGGL((cm::bm_kernel<4, false>), grid, dim3(256), 0, stream, *a); break;
+    case 9: hipLaunchKernelGGL((cm::bm_kernel<4, true>), grid, dim3(256), 0, stream, *a); break;
     default: return -1;
   }
   return hipGetLastError() == hipSuccess ? 0 : -2;

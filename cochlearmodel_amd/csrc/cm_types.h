@@ -32,6 +32,10 @@ typedef struct {
   int32_t j_slow;        // JACOBBY_Loops_Slow (passes 0, 1)
   int32_t max_passes;    // Cuda_Outern_Loops
   int32_t decision_mode; // 0: decision from the full reduction; 1: zero-partial lane outcome
+  int32_t jacobi_mode;   // 0: fixed sweeps (reference); 1: sweep until the relative residual <= tol_rel
+  int32_t check_every;   // tolerance mode: sweeps between residual checks
+  int32_t max_sweeps;    // tolerance mode: sweep cap per pass
+  float tol_rel;         // tolerance mode: |Y - A p|_2 <= tol_rel |Y|_2
 } cm_bm_scalars;
 
 // One time block (or one whole signal in continuous mode).
@@ -46,8 +50,10 @@ typedef struct {
   int32_t pad;
 } cm_bm_item;
 
-// Per-section packed parameters: 8 x 256 floats in this order.
-enum { CM_SEC_MASS = 0, CM_SEC_RSM, CM_SEC_R, CM_SEC_S, CM_SEC_Q, CM_SEC_SOHC, CM_SEC_STM, CM_SEC_RTM, CM_SEC_COUNT };
+// Per-section packed parameters: CM_SEC_COUNT x 256 floats in this order.
+// CM_SEC_M is the tridiagonal's diagonal as float (used by the residual test).
+enum { CM_SEC_MASS = 0, CM_SEC_RSM, CM_SEC_R, CM_SEC_S, CM_SEC_Q, CM_SEC_SOHC, CM_SEC_STM, CM_SEC_RTM, CM_SEC_M,
+       CM_SEC_COUNT };
 
 // Per-block statistics written by the BM kernel.
 typedef struct {

@@ -223,8 +223,10 @@ int BMOHCKernel_Init(const float *gamma, const float *mass, const float *M, cons
   }
   if (g_bm.sec.ensure(CM_SEC_COUNT * CM_SECTIONS) || g_bm.gamma.ensure(CM_SECTIONS)) return -1;
   float host[CM_SEC_COUNT * CM_SECTIONS];
-  const float *src[CM_SEC_COUNT] = {mass, M, R, S, Q, S_ohc, S_tm, R_tm};
-  for (int k = 0; k < CM_SEC_COUNT; ++k) std::memcpy(host + k * CM_SECTIONS, src[k], CM_SECTIONS * sizeof(float));
+  const float *src[CM_SEC_M] = {mass, M, R, S, Q, S_ohc, S_tm, R_tm};
+  for (int k = 0; k < CM_SEC_M; ++k) std::memcpy(host + k * CM_SECTIONS, src[k], CM_SECTIONS * sizeof(float));
+  // the diagonal itself (only the tolerance-mode residual uses it)
+  for (int i = 0; i < CM_SECTIONS; ++i) host[CM_SEC_M * CM_SECTIONS + i] = (float)(1.0 / (double)M[i]);
   if (check(hipMemcpy(g_bm.sec.ptr, host, sizeof(host), hipMemcpyHostToDevice), "upload section params")) return -1;
   if (check(hipMemcpy(g_bm.gamma.ptr, gamma, CM_SECTIONS * sizeof(float), hipMemcpyHostToDevice), "upload gamma"))
     return -1;
@@ -289,6 +291,7 @@ int BMOHCNewKernel(const float *input_samples, int override_input_samples, float
     if (check(hipMemset(g_bm.input.ptr + samplesBufferLengthP1, 0, 2 * sizeof(float)), "pad input")) return -1;
   }
   cm_bm_scalars sc;
+  std::memset(&sc, 0, sizeof(sc));
   sc.w_ohc = w_ohc;
   sc.time_step = time_step;
   sc.time_step_out = time_step_out;

@@ -72,7 +72,8 @@ int cm_tri_matrix(double *lower, double *mid, double *upper) {
   return 0;
 }
 
-// 8 x 256 floats: mass, 1/m_i, R, S, Q, S_ohc, S_tm, R_tm (src/solver.cpp:357-376).
+// CM_SEC_COUNT x 256 floats: mass, 1/m_i, R, S, Q, S_ohc, S_tm, R_tm (src/solver.cpp:357-376)
+// and the diagonal m_i itself.
 int cm_pack_section_params(float *out) {
   if (!out) return -3;
   double v[7 * NS], m[NS];
@@ -87,6 +88,7 @@ int cm_pack_section_params(float *out) {
     out[CM_SEC_SOHC * NS + i] = (float)v[5 * NS + i];
     out[CM_SEC_STM * NS + i] = (float)v[4 * NS + i];
     out[CM_SEC_RTM * NS + i] = (float)v[3 * NS + i];
+    out[CM_SEC_M * NS + i] = (float)m[i];
   }
   return 0;
 }
@@ -118,6 +120,10 @@ int cm_default_bm_scalars(float Fs, cm_bm_scalars *o) {
   o->j_slow = 20;
   o->max_passes = 10;
   o->decision_mode = 0;
+  o->jacobi_mode = 0;
+  o->check_every = 4;
+  o->max_sweeps = 1000;
+  o->tol_rel = 1e-4f;
   return 0;
 }
 

@@ -32,7 +32,7 @@ def tri_matrix():
 
 
 def section_params() -> np.ndarray:
-    out = np.zeros((8, SECTIONS), np.float32)
+    out = np.zeros((9, SECTIONS), np.float32)
     check(lib().cm_pack_section_params(ptr(out)), "cm_pack_section_params")
     return out
 

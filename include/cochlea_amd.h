@@ -213,10 +213,32 @@ int cm_model_vectors(double *out7x256);
 /* replaces CSolver::Init_Tri_Matrix (src/solver.cpp:155-176); any may be NULL */
 int cm_tri_matrix(double *lower, double *mid, double *upper);
 /* replaces the per-section packing of Run_Cuda (src/solver.cpp:357-376):
- * 8 x 256 floats in cm_types.h CM_SEC_* order */
-int cm_pack_section_params(float *out8x256);
+ * CM_SEC_COUNT x 256 floats in cm_types.h CM_SEC_* order */
+int cm_pack_section_params(float *out);
 /* scalar arguments Run_Cuda passes to BMOHCNewKernel (src/solver.cpp:629-659) */
 int cm_default_bm_scalars(float Fs, cm_bm_scalars *out);
+
+/* ---- learned warm start (no reference counterpart) ----------------------
+ * Kernels of the 769 -> 512 -> 512 -> 256 GELU pressure-guess MLP.  Matrices
+ * are row-major; bf16 buffers are passed as void* (uint16 bit patterns).
+ * C[M x N] (fp32) = op(A) op(B) (+ bias[N]) (+ beta * C), op = transpose when
+ * the flag is set; BF16 MFMA with FP32 accumulation. */
+int cm_gemm_bf16(int transA, int transB, int M, int N, int K, const void *A, int lda, const void *B, int ldb,
+                 float *C, int ldc, const float *bias, float beta);
+/* G = gelu(Z) (tanh form) to bf16 and/or fp32 (either output may be NULL) */
+int cm_gelu_fwd(const float *Z, void *Gbf, float *Gf, int64_t n);
+/* dZ = dG * gelu'(Z) to bf16 and/or fp32 */
+int cm_gelu_bwd(const float *dG, const float *Z, void *dZbf, float *dZf, int64_t n);
+/* fp32 -> bf16, round to nearest even */
+int cm_to_bf16(const float *x, void *y, int64_t n);
+/* db[n] = sum over rows of dZ[M x N] */
+int cm_colsum(const float *dZ, float *db, int M, int N);
+/* dY = 2 (Y - T) / (M N) (fp32 and bf16); row_loss[m] = sum_n (Y - T)^2 */
+int cm_mse_grad(const float *Y, const float *T, float *dY, void *dYbf, float *row_loss, int M, int N);
+/* Adam step on fp32 master weights w (+ decoupled-free L2 term wd * w in the
+ * gradient); refreshes the bf16 copy wbf when it is not NULL */
+int cm_adam(float *w, void *wbf, const float *g, float *m1, float *m2, int64_t n, float lr, float b1, float b2,
+            float eps, int step, float wd);
 
 #ifdef __cplusplus
 }

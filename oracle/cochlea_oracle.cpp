@@ -110,6 +110,26 @@ template <typename T> T tree_max(const T *x) {
   return m;
 }
 
+// Exact solve of the kernel-convention tridiagonal (row i: low_i p_{i-1} +
+// m_i p_i + p_{i+1} = Y_i, p_{-1} = p_256 = 0) by the Thomas algorithm in FP64.
+template <typename T> void thomas_exact(const T *Y, const T *mf, const T *low, T *p) {
+  double c[NS], d[NS];
+  for (int i = 0; i < NS; ++i) {
+    const double a = i > 0 ? (double)low[i] : 0.0;
+    const double b = (double)mf[i];
+    const double up = i < NS - 1 ? 1.0 : 0.0;
+    const double den = i > 0 ? b - a * c[i - 1] : b;
+    c[i] = up / den;
+    d[i] = ((double)Y[i] - (i > 0 ? a * d[i - 1] : 0.0)) / den;
+  }
+  double x = d[NS - 1];
+  p[NS - 1] = (T)x;
+  for (int i = NS - 2; i >= 0; --i) {
+    x = d[i] - c[i] * x;
+    p[i] = (T)x;
+  }
+}
+
 // ------------------------------------------------------ kernel #17 -------
 template <typename T> struct Outs {
   T *speeds;
@@ -126,7 +146,7 @@ template <typename T>
 void bm_item(const orc_bm_scalars &sc, const float *sec, const float *gamma_set, const orc_bm_item &it,
              int item_index, const float *input, int64_t input_len, const Outs<T> &o) {
   // per-section parameters (:13700-13736); promoted from the packed FP32 values
-  T rsm[NS], rmass[NS], gam[NS], R[NS], S[NS], Q[NS], Sohc[NS], Stm[NS], Rtm[NS], low[NS];
+  T rsm[NS], rmass[NS], gam[NS], R[NS], S[NS], Q[NS], Sohc[NS], Stm[NS], Rtm[NS], low[NS], mf[NS];
   for (int i = 0; i < NS; ++i) {
     T mass = (T)sec[0 * NS + i];
     rsm[i] = (T)sec[1 * NS + i];
@@ -136,6 +156,7 @@ void bm_item(const orc_bm_scalars &sc, const float *sec, const float *gamma_set,
     Sohc[i] = (T)sec[5 * NS + i];
     Stm[i] = (T)sec[6 * NS + i];
     Rtm[i] = (T)sec[7 * NS + i];
+    mf[i] = (T)sec[8 * NS + i];
     gam[i] = (T)gamma_set[i];
     rmass[i] = (T)1 / mass;                 // reciprocal_mass_ind = 1.0f / mass[i]
     low[i] = i < NS - 2 ? (T)1 : (T)0;      // low_multiplier, SECTIONSM2
@@ -223,16 +244,45 @@ void bm_item(const orc_bm_scalars &sc, const float *sec, const float *gamma_set,
       Y[0] = dxa0 * ((((-a2) * ow_nv) - a1 * ow_nd) + Gme * s);
       Y[NS - 1] = 0;
       // Jacobi sweeps, warm start from the resident pressure (:13962-13982)
-      int J = out_loop < 2 ? sc.j_slow : sc.j_fast;
-      for (int it2 = 0; it2 < J; ++it2) {
-        for (int i = 0; i < NS; ++i) {
-          T pm1 = i > 0 ? p[i - 1] : (T)0;
-          T pp1 = i < NS - 1 ? p[i + 1] : (T)0;
-          T tmp = low[i] * pm1 + pp1;
-          tmp = Y[i] - tmp;
-          pn[i] = tmp * rsm[i];
+      auto sweep = [&](int count) {
+        for (int it2 = 0; it2 < count; ++it2) {
+          for (int i = 0; i < NS; ++i) {
+            T pm1 = i > 0 ? p[i - 1] : (T)0;
+            T pp1 = i < NS - 1 ? p[i + 1] : (T)0;
+            T tmp = low[i] * pm1 + pp1;
+            tmp = Y[i] - tmp;
+            pn[i] = tmp * rsm[i];
+          }
+          std::memcpy(p, pn, sizeof(T) * NS);
         }
-        std::memcpy(p, pn, sizeof(T) * NS);
+      };
+      int J;
+      if (sc.jacobi_mode == 2) {             // O2: exact FP64 pressure of this pass
+        thomas_exact(Y, mf, low, p);
+        J = 0;
+      } else if (sc.jacobi_mode == 1) {      // sweep until |Y - A p| <= tol |Y|
+        T y2[NS], r2[NS];
+        for (int i = 0; i < NS; ++i) y2[i] = Y[i] * Y[i];
+        const T yy = tree_sum(y2);
+        const T tau2 = (T)sc.tol_rel * (T)sc.tol_rel;
+        J = 0;
+        for (;;) {
+          for (int i = 0; i < NS; ++i) {
+            T pm1 = i > 0 ? p[i - 1] : (T)0;
+            T pp1 = i < NS - 1 ? p[i + 1] : (T)0;
+            T off = low[i] * pm1 + pp1;
+            T r = Y[i] - (off + mf[i] * p[i]);
+            r2[i] = r * r;
+          }
+          const T rr = tree_sum(r2);
+          if (rr <= tau2 * yy || J >= sc.max_sweeps) break;
+          const int k = std::min(sc.check_every, sc.max_sweeps - J);
+          sweep(k);
+          J += k;
+        }
+      } else {
+        J = out_loop < 2 ? sc.j_slow : sc.j_fast;
+        sweep(J);
       }
       n_sweeps += J;
       for (int i = 0; i < NS; ++i) na[i] = (p[i] + G[i]) * rmass[i];   // (:13989-13991)
@@ -401,6 +451,10 @@ int orc_default_scalars(float Fs, orc_bm_scalars *o) {
   o->j_slow = 20;
   o->max_passes = 10;
   o->decision_mode = 0;
+  o->jacobi_mode = 0;
+  o->check_every = 4;
+  o->max_sweeps = 1000;
+  o->tol_rel = 1e-4f;
   return 0;
 }
 

@@ -44,6 +44,10 @@ typedef struct {
   int32_t j_slow;       /* JACOBBY_Loops_Slow */
   int32_t max_passes;   /* Cuda_Outern_Loops */
   int32_t decision_mode;/* 0 = lane-0 full reduction, 1 = zero-lane race outcome */
+  int32_t jacobi_mode;  /* 0 = fixed sweeps (reference), 1 = to tolerance, 2 = exact FP64 Thomas solve (O2) */
+  int32_t check_every;  /* tolerance mode: sweeps between residual checks */
+  int32_t max_sweeps;   /* tolerance mode: sweep cap per pass */
+  float tol_rel;        /* tolerance mode: stop when |Y - A p| <= tol_rel |Y| */
 } orc_bm_scalars;
 
 /* One work item = one reference "time block" (src/cochlea.cu:1148-1176). */
@@ -71,8 +75,8 @@ int orc_tri_mid_diag(double *m256);
 int orc_block_decomposition(int time_blocks, int bufP1, int overlap, int decouple_filter,
                             int32_t *start, int32_t *mosi, int32_t *mesi, int32_t *write_start);
 
-/* Kernel-17 restatement.  sec_f32 = 8 x 256 floats (mass, rsm, R, S, Q,
- * S_ohc, S_tm, R_tm); gamma = n_sets x 256.  Outputs may be NULL. */
+/* Kernel-17 restatement.  sec_f32 = 9 x 256 floats (mass, rsm, R, S, Q,
+ * S_ohc, S_tm, R_tm, m = diagonal as float); gamma = n_sets x 256.  Outputs may be NULL. */
 int orc_bm_run_f32(const orc_bm_scalars *sc, const float *sec_f32, const float *gamma,
                    const orc_bm_item *items, int n_items, const float *input, int64_t input_len,
                    float *speeds, int64_t rows,

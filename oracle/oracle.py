@@ -25,7 +25,8 @@ class BMScalars(C.Structure):
         "w_ohc", "time_step", "time_step_out", "delta_x", "alpha_r", "Ts", "alpha_l", "alpha_s",
         "Gme", "a0", "a1", "a2", "sigma_ow", "eta1", "eta2", "dt_min", "dt_max")] + [
         ("j_fast", C.c_int32), ("j_slow", C.c_int32), ("max_passes", C.c_int32),
-        ("decision_mode", C.c_int32)]
+        ("decision_mode", C.c_int32), ("jacobi_mode", C.c_int32), ("check_every", C.c_int32),
+        ("max_sweeps", C.c_int32), ("tol_rel", C.c_float)]
 
 
 ITEM_DTYPE = np.dtype([("start", np.int32), ("mosi", np.int32), ("mesi", np.int32),
@@ -81,10 +82,10 @@ def default_scalars(Fs: float = 44100.0) -> BMScalars:
 
 
 def section_params() -> np.ndarray:
-    """8 x 256 float32: mass, 1/m_i, R, S, Q, S_ohc, S_tm, R_tm (src/solver.cpp:357-376)."""
+    """9 x 256 float32: mass, 1/m_i, R, S, Q, S_ohc, S_tm, R_tm (src/solver.cpp:357-376), m_i."""
     v = model_vectors()
     m = tri_mid_diag()
-    out = np.zeros((8, SECTIONS), np.float32)
+    out = np.zeros((9, SECTIONS), np.float32)
     out[0] = v[0].astype(np.float32)
     out[1] = np.float32(1.0) / m.astype(np.float32)
     out[2] = v[1].astype(np.float32)
@@ -93,6 +94,7 @@ def section_params() -> np.ndarray:
     out[5] = v[5].astype(np.float32)
     out[6] = v[4].astype(np.float32)
     out[7] = v[3].astype(np.float32)
+    out[8] = m.astype(np.float32)
     return out
 
 
