@@ -90,7 +90,16 @@ class C2Workload:
         self.d_scratch = torch.empty((2, self.rows, SECTIONS), dtype=torch.float32, device=dev)
         self.stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
 
-    def step(self, timed_bm: bool = True):
+    def run_bm(self, sc, timed: bool = True):
+        C, L, lib = self.C, self.L, self._lib
+        L.cm_set_stream(self.stream)
+        lib.check(L.cm_bm_run(C.byref(sc), lib.ptr(self.d_input), self.d_input.numel(), lib.ptr(self.bm.sec),
+                              lib.ptr(self.bm.gamma), lib.ptr(self.d_items), self.TB, lib.ptr(self.d_speeds),
+                              self.bufP1, None, None, None, None, None, None, None, None, lib.ptr(self.d_stats), 1, 0,
+                              1 if timed else 0), "bm")
+        return L.cm_last_kernel_ms() if timed else None
+
+    def step(self, sc, timed_bm: bool = True):
         torch, C, L, lib = self.torch, self.C, self.L, self._lib
         L.cm_set_stream(self.stream)
         t = {}
@@ -100,12 +109,9 @@ class C2Workload:
                                           self.norms[0], self.norms[1], 0, self.norms[2], self.norms[3], 0, self.n,
                                           self.interval, self.head, C.c_float(FS)), "input")
         e1.record()
-        lib.check(L.cm_bm_run(C.byref(self.sc), lib.ptr(self.d_input), self.d_input.numel(), lib.ptr(self.bm.sec),
-                              lib.ptr(self.bm.gamma), lib.ptr(self.d_items), self.TB, lib.ptr(self.d_speeds),
-                              self.bufP1, None, None, None, None, None, None, None, None, lib.ptr(self.d_stats), 1, 0,
-                              1 if timed_bm else 0), "bm")
+        ms = self.run_bm(sc, timed_bm)
         if timed_bm:
-            t["bm_ms"] = L.cm_last_kernel_ms()
+            t["bm_ms"] = ms
         e2.record()
         self.ihc.run(self.d_speeds, self.rows, self.fir_ivs, d_lambda=self.d_lambda, d_scratch=self.d_scratch)
         e3 = torch.cuda.Event(enable_timing=True)
@@ -120,16 +126,64 @@ class C2Workload:
         return int(s[:, 0].sum()), int(s[:, 1].sum()), int(s[:, 2].sum())
 
 
-def cpu_baseline(work: C2Workload, n_blocks: int, threads: int):
-    """The CPU oracle (FP32 port of kernel 17) on every (TB/n_blocks)-th block."""
-    from oracle import oracle as O
-    inp = work.d_input.cpu().numpy()
+TOL = 1e-6   # relative residual of the tolerance-mode Jacobi
+
+
+def variant_scalars(name: str):
+    """Solver settings of each measured variant (cm_bm_scalars)."""
+    from cochlearmodel_amd import solver
+    sc = solver.default_scalars(FS)
+    if name == "reference_20_2":
+        return sc
+    sc.jacobi_mode, sc.tol_rel, sc.check_every, sc.max_sweeps = 1, TOL, 4, 1000
+    sc.warm_start = {"plain_tol": 0, "extrapolated_tol": 1}[name]
+    return sc
+
+
+VARIANTS = ("reference_20_2", "plain_tol", "extrapolated_tol")
+
+
+def sample_blocks(work, n_blocks):
     stride = max(1, work.TB // n_blocks)
-    sel = np.arange(0, work.TB, stride)[:n_blocks]
+    return np.arange(0, work.TB, stride)[:n_blocks]
+
+
+def oracle_items(work, sel):
+    from oracle import oracle as O
     it = np.zeros(len(sel), O.ITEM_DTYPE)
     for f in ("start", "mosi", "mesi", "write_start", "tol", "m1tol", "set"):
         it[f] = work.items[f][sel]
+    return it
+
+
+def block_rows(work, sel):
+    ws, me = work.items["write_start"][sel], work.items["mesi"][sel]
+    return np.concatenate([np.arange(a, b) for a, b in zip(ws, me)])
+
+
+def exact_reference(work, n_blocks, threads):
+    """O2 (exact per-pass pressure solve, FP32 state) on sampled blocks: the
+    accuracy yardstick every variant is scored against."""
+    from oracle import oracle as O
+    sel = sample_blocks(work, n_blocks)
     sc = O.default_scalars(FS)
+    sc.jacobi_mode = 2
+    o = O.bm_run(sc, O.section_params(), np.full(SECTIONS, 0.5, np.float32), oracle_items(work, sel),
+                 work.d_input.cpu().numpy(), work.bufP1, "f32", threads=threads)
+    rows = block_rows(work, sel)
+    return rows, o["speeds"][rows]
+
+
+def cpu_baseline(work: C2Workload, n_blocks: int, threads: int, sc_gpu):
+    """The CPU oracle (FP32 port of kernel 17) on every (TB/n_blocks)-th block."""
+    from oracle import oracle as O
+    inp = work.d_input.cpu().numpy()
+    sel = sample_blocks(work, n_blocks)
+    stride = max(1, work.TB // n_blocks)
+    it = oracle_items(work, sel)
+    sc = O.default_scalars(FS)
+    for f, _ in O.BMScalars._fields_:
+        setattr(sc, f, getattr(sc_gpu, f))
     t0 = time.perf_counter()
     o = O.bm_run(sc, O.section_params(), np.full(SECTIONS, 0.5, np.float32), it, inp, work.bufP1, "f32",
                  threads=threads)
@@ -137,7 +191,7 @@ def cpu_baseline(work: C2Workload, n_blocks: int, threads: int):
     steps = int(o["steps"].sum())
     return {"value": SECTIONS * steps / dt, "unit": "section-steps/s", "cores": threads, "kind": "port",
             "sample": f"{len(sel)} of {work.TB} time blocks (every {stride}th), {steps} committed steps, "
-                      f"{dt:.2f} s wall, FP32 oracle of kernel 17 with OpenMP over blocks"}, o, sel
+                      f"{dt:.2f} s wall, FP32 oracle of the same solver settings with OpenMP over blocks"}, o, sel
 
 
 def main():
@@ -146,9 +200,12 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--signals", type=int, default=256)
+    ap.add_argument("--variant", default="extrapolated_tol", choices=VARIANTS)
     ap.add_argument("--cpu-blocks", type=int, default=512)
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--accuracy-blocks", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-variants", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "only one GPU is used: runs on several GPUs are not supported",
@@ -157,18 +214,22 @@ def main():
     import torch
     torch.cuda.set_device(0)
     work = C2Workload(n_signals=args.signals)
+    sc = variant_scalars(args.variant)
+    print(f"# workload ready: {work.n} signals, {work.TB} time blocks, variant {args.variant}", file=sys.stderr,
+          flush=True)
     for _ in range(args.warmup):
-        work.step()
+        work.step(sc)
     torch.cuda.synchronize()
     bm_ms, ihc_ms, in_ms = [], [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        t = work.step()
+        t = work.step(sc)
         bm_ms.append(t["bm_ms"])
         ihc_ms.append(t["ihc_ms"])
         in_ms.append(t["input_ms"])
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    print(f"# timed {args.steps} steps in {wall:.2f} s", file=sys.stderr, flush=True)
     steps, passes, sweeps = work.stats()
     ms_per_step = 1e3 * wall / args.steps
     sec_steps = SECTIONS * steps
@@ -177,35 +238,57 @@ def main():
     flops = flops_per_launch(steps, passes, sweeps)
     achieved = flops / (bm_avg * 1e-3) / 1e12
     ihc_bytes = work.rows * SECTIONS * 16.0        # read BM, write 3 lambdas
+    head_speeds = work.d_speeds.cpu().numpy()
+    desc = {"reference_20_2": "reference kernel-17 method (fixed 20/2 Jacobi sweeps)",
+            "plain_tol": f"Jacobi to relative residual {TOL:g}, previous-pressure guess",
+            "extrapolated_tol": f"Jacobi to relative residual {TOL:g}, extrapolated predictor-pass guess with "
+                                "residual gate"}
     out = {
         "metric": "section-steps/s + Jacobi iters/step: learned warm start vs plain @1e-4 rel err",
         "value": value, "unit": "section-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": "256 tones (32 freq 250-8000 Hz x 8 levels 20-90 dB), 200 ms at 44.1 kHz, "
-                               "256 sections, Decouple_Filter=10, 2560 time blocks, plain kernel-17 method "
-                               "(20/2 Jacobi sweeps), input synthesis + BM + IHC/lambda per step",
-                   "signals": work.n, "time_blocks": work.TB, "rows": work.rows},
+                               "256 sections, Decouple_Filter=10, 2560 time blocks, input synthesis + BM + "
+                               "IHC/lambda per step; BM solver: " + desc[args.variant],
+                   "variant": args.variant, "signals": work.n, "time_blocks": work.TB, "rows": work.rows},
         "jacobi_iters_per_step": sweeps / steps, "passes_per_step": passes / steps,
         "committed_steps": steps, "signals_per_s": work.n / (ms_per_step * 1e-3),
         "bm_ms": bm_avg, "ihc_lambda_ms": float(np.mean(ihc_ms)), "input_ms": float(np.mean(in_ms)),
         "bm_section_steps_per_s": sec_steps / (bm_avg * 1e-3),
         "roofline": {"bound": "mfma", "engine": "FP32 VALU", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                     "kernel": "bm_kernel<1>", "flop_per_launch": flops},
+                     "kernel": "bm_kernel", "flop_per_launch": flops},
         "ihc_lambda_roofline": {"bound": "hbm", "achieved": ihc_bytes / (np.mean(ihc_ms) * 1e-3) / 1e9,
                                 "peak": PEAK_HBM_GBS, "unit": "GB/s",
                                 "frac": ihc_bytes / (np.mean(ihc_ms) * 1e-3) / 1e9 / PEAK_HBM_GBS},
     }
+    threads = min(args.cpu_threads, os.cpu_count() or 1)
+    if not args.no_variants:
+        # every solver variant on the same resident input: BM kernel time, work and accuracy
+        rows, exact = exact_reference(work, args.accuracy_blocks, threads)
+        print(f"# exact-solve reference on {args.accuracy_blocks} blocks done", file=sys.stderr, flush=True)
+        nrm = max(float(np.linalg.norm(exact)), 1e-30)
+        var = {}
+        for name in VARIANTS:
+            vsc = variant_scalars(name)
+            ms = work.run_bm(vsc, True)
+            torch.cuda.synchronize()
+            st, pa, sw = work.stats()
+            v = work.d_speeds.cpu().numpy()[rows]
+            var[name] = {"bm_ms": ms, "bm_section_steps_per_s": SECTIONS * st / (ms * 1e-3),
+                         "jacobi_iters_per_step": sw / st, "passes_per_step": pa / st, "committed_steps": st,
+                         "rel_l2_vs_exact_solve": float(np.linalg.norm(v - exact) / nrm)}
+            print(f"# variant {name}: {ms:.1f} ms, {sw / st:.2f} sweeps/step", file=sys.stderr, flush=True)
+        out["variants"] = var
+        out["accuracy_sample"] = (f"BM velocity of {args.accuracy_blocks} sampled time blocks ({len(rows)} rows) "
+                                  "against the CPU oracle's exact per-pass pressure solve (FP32 state)")
     if not args.no_cpu:
-        threads = min(args.cpu_threads, os.cpu_count() or 1)
-        cb, o, sel = cpu_baseline(work, args.cpu_blocks, threads)
+        cb, o, sel = cpu_baseline(work, args.cpu_blocks, threads, sc)
         out["cpu_baseline"] = cb
         # the GPU speeds of the sampled blocks agree with the oracle
-        v = work.d_speeds.cpu().numpy()
-        ws, me = work.items["write_start"][sel], work.items["mesi"][sel]
-        rows = np.concatenate([np.arange(a, b) for a, b in zip(ws, me)])
-        d = np.linalg.norm(v[rows] - o["speeds"][rows]) / max(np.linalg.norm(o["speeds"][rows]), 1e-30)
+        rows = block_rows(work, sel)
+        d = np.linalg.norm(head_speeds[rows] - o["speeds"][rows]) / max(np.linalg.norm(o["speeds"][rows]), 1e-30)
         out["cpu_baseline"]["parity_rel_l2"] = float(d)
     print(json.dumps(out))
 

@@ -119,10 +119,22 @@ __device__ __forceinline__ void jacobi_sweeps(const float (&Y)[4], const float (
   }
 }
 
+// Squared 2-norm of the residual Y - A p (kernel convention, fixed tree).
+__device__ __forceinline__ float resid2(const float (&Y)[4], const float (&mf)[4], float low23, const float (&p)[4]) {
+  const float left = wave_shr1(p[3]);
+  const float right = wave_shl1(p[0]);
+  const float r0 = Y[0] - ((left + p[1]) + mf[0] * p[0]);
+  const float r1 = Y[1] - ((p[0] + p[2]) + mf[1] * p[1]);
+  const float r2 = Y[2] - ((low23 * p[1] + p[3]) + mf[2] * p[2]);
+  const float r3 = Y[3] - ((low23 * p[2] + right) + mf[3] * p[3]);
+  return tree_sum(r0 * r0, r1 * r1, r2 * r2, r3 * r3);
+}
+
 // One time block, include/cochlea.cuh:13620-14232.  TOL selects the
 // sweep-to-tolerance Jacobi; otherwise the reference's fixed 20/2 sweeps.
-template <bool TOL>
-__device__ void run_item(const BmLaunch &a, int item_index, int lane) {
+// WS = 1 adds the extrapolated predictor-pass guess (tolerance mode only).
+template <bool TOL, int WS>
+__device__ void run_item(const BmLaunch &a, int item_index, int lane, float4 *pold_lds) {
   const cm_bm_item it = a.items[item_index];
   const cm_bm_scalars &sc = a.sc;
   const int base = 4 * lane;
@@ -173,9 +185,14 @@ __device__ void run_item(const BmLaunch &a, int item_index, int lane) {
   float nd[4] = {0, 0, 0, 0}, nv[4] = {0, 0, 0, 0}, na[4] = {0, 0, 0, 0}, npsi[4] = {0, 0, 0, 0};
   float npsid[4] = {0, 0, 0, 0}, nxis[4] = {0, 0, 0, 0}, nxi[4] = {0, 0, 0, 0}, Ptm[4] = {0, 0, 0, 0};
   float p[4] = {0, 0, 0, 0};
+  // previous step's pressure (WS == 1) lives in LDS: it is touched once per step
+  if (WS == 1) pold_lds[lane] = make_float4(0.f, 0.f, 0.f, 0.f);
   float ow_pd = 0, ow_pv = 0, ow_pa = 0, ow_nd = 0, ow_nv = 0, ow_na = 0;
 
   float dt = sc.time_step;
+  // dt of the previous step (WS == 1), kept in the spare LDS slot after the pressures
+  float *dt_prev_lds = reinterpret_cast<float *>(pold_lds + 64);
+  if (WS == 1 && lane == 0) *dt_prev_lds = dt;
   float t = 0.0f;
   int in_s = it.start;
   float sx0 = in_at(a, in_s), sx1 = in_at(a, (int64_t)in_s + 1);
@@ -246,15 +263,28 @@ __device__ void run_item(const BmLaunch &a, int item_index, int lane) {
         // sweep until |Y - A p|_2 <= tol |Y|_2, checking every check_every sweeps
         const float yy = tree_sum(Y[0] * Y[0], Y[1] * Y[1], Y[2] * Y[2], Y[3] * Y[3]);
         const float tau2 = sc.tol_rel * sc.tol_rel;
+        if (WS == 1 && out_loop == 0) {
+          // predictor-pass guess: keep the linear extrapolation of the last
+          // two steps' pressures when its residual is smaller
+          // (in place: p is parked in LDS while the extrapolation is scored)
+          const float rho = dt / *dt_prev_lds;
+          const float rr0 = resid2(Y, mf, low23, p);
+          const float4 po = pold_lds[lane];
+          pold_lds[lane] = make_float4(p[0], p[1], p[2], p[3]);
+          p[0] = p[0] + rho * (p[0] - po.x);
+          p[1] = p[1] + rho * (p[1] - po.y);
+          p[2] = p[2] + rho * (p[2] - po.z);
+          p[3] = p[3] + rho * (p[3] - po.w);
+          const float rre = resid2(Y, mf, low23, p);
+          if (!(rre < rr0)) {
+            const float4 pk = pold_lds[lane];
+            p[0] = pk.x; p[1] = pk.y; p[2] = pk.z; p[3] = pk.w;
+          }
+          if (lane == 0) *dt_prev_lds = dt;
+        }
         J = 0;
         for (;;) {
-          const float left = wave_shr1(p[3]);
-          const float right = wave_shl1(p[0]);
-          const float r0 = Y[0] - ((left + p[1]) + mf[0] * p[0]);
-          const float r1 = Y[1] - ((p[0] + p[2]) + mf[1] * p[1]);
-          const float r2 = Y[2] - ((low23 * p[1] + p[3]) + mf[2] * p[2]);
-          const float r3 = Y[3] - ((low23 * p[2] + right) + mf[3] * p[3]);
-          const float rr = tree_sum(r0 * r0, r1 * r1, r2 * r2, r3 * r3);
+          const float rr = resid2(Y, mf, low23, p);
           if (rr <= tau2 * yy || J >= sc.max_sweeps) break;
           const int kk = sc.check_every < sc.max_sweeps - J ? sc.check_every : sc.max_sweeps - J;
           jacobi_sweeps(Y, rsm, low23, p, kk);
@@ -359,12 +389,13 @@ __device__ void run_item(const BmLaunch &a, int item_index, int lane) {
   }
 }
 
-template <int WAVES, bool TOL>
-__global__ __launch_bounds__(64 * WAVES) void bm_kernel(BmLaunch a) {
+template <int WAVES, bool TOL, int WS>
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(3))) void bm_kernel(BmLaunch a) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int idx = blockIdx.x * WAVES + wave;
-  if (idx < a.n_items) run_item<TOL>(a, idx, lane);
+  __shared__ float4 pold_lds[WS ? WAVES * 65 : 1];
+  if (idx < a.n_items) run_item<TOL, WS>(a, idx, lane, pold_lds + (WS ? wave * 65 : 0));
 }
 
 
@@ -375,14 +406,24 @@ extern "C" int cm_launch_bm(const cm::BmLaunch *a, int waves_per_block, int grid
   if (a->use_queue) return -1;   // the work-queue launch is switched off
   dim3 grid(grid_blocks);
   const bool tol = a->sc.jacobi_mode == 1;
-  switch (waves_per_block * 2 + (tol ? 1 : 0)) {
-    case 2: hipLaunchKernelGGL((cm::bm_kernel<1, false>), grid, dim3(64), 0, stream, *a); break;
-    case 3: hipLaunchKernelGGL((cm::bm_kernel<1, true>), grid, dim3(64), 0, stream, *a); break;
-    case 4: hipLaunchKernelGGL((cm::bm_kernel<2, false>), grid, dim3(128), 0, stream, *a); break;
-    case 5: hipLaunchKernelGGL((cm::bm_kernel<2, true>), grid, dim3(128), 0, stream, *a); break;
-    case 8: hipLaunchKernelGGL((cm::bm_kernel<4, false>), grid, dim3(256), 0, stream, *a); break;
-    case 9: hipLaunchKernelGGL((cm::bm_kernel<4, true>), grid, dim3(256), 0, stream, *a); break;
+  const int ws = tol ? a->sc.warm_start : 0;
+  if (ws < 0 || ws > 1) return -1;
+#define CM_BM_CASE(W, T, S) \
+  case (W) * 4 + (T) * 2 + (S): \
+    hipLaunchKernelGGL((cm::bm_kernel<W, T, S>), grid, dim3(64 * (W)), 0, stream, *a); \
+    break;
+  switch (waves_per_block * 4 + (tol ? 2 : 0) + ws) {
+    CM_BM_CASE(1, false, 0)
+    CM_BM_CASE(1, true, 0)
+    CM_BM_CASE(1, true, 1)
+    CM_BM_CASE(2, false, 0)
+    CM_BM_CASE(2, true, 0)
+    CM_BM_CASE(2, true, 1)
+    CM_BM_CASE(4, false, 0)
+    CM_BM_CASE(4, true, 0)
+    CM_BM_CASE(4, true, 1)
     default: return -1;
   }
+#undef CM_BM_CASE
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }

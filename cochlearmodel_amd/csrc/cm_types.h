@@ -36,6 +36,9 @@ typedef struct {
   int32_t check_every;   // tolerance mode: sweeps between residual checks
   int32_t max_sweeps;    // tolerance mode: sweep cap per pass
   float tol_rel;         // tolerance mode: |Y - A p|_2 <= tol_rel |Y|_2
+  int32_t warm_start;    // tolerance mode, predictor pass guess: 0 previous pressure; 1 the better (by
+                         // residual, ties to the previous pressure) of it and the linear extrapolation
+                         // p + (dt/dt_prev)(p - p_prev) from the last two steps' pressures
 } cm_bm_scalars;
 
 // One time block (or one whole signal in continuous mode).

@@ -124,6 +124,7 @@ int cm_default_bm_scalars(float Fs, cm_bm_scalars *o) {
   o->check_every = 4;
   o->max_sweeps = 1000;
   o->tol_rel = 1e-4f;
+  o->warm_start = 0;
   return 0;
 }
 

@@ -171,16 +171,18 @@ void bm_item(const orc_bm_scalars &sc, const float *sec, const float *gamma_set,
   const T dxdx = dx * dx;
   const T dxa0 = dx * a0;
 
-  std::vector<T> st(26 * NS, (T)0);
+  std::vector<T> st(28 * NS, (T)0);
   T *pd = &st[0 * NS], *pv = &st[1 * NS], *pa = &st[2 * NS], *ppsi = &st[3 * NS], *ppsid = &st[4 * NS];
   T *pxis = &st[5 * NS], *pxi = &st[6 * NS];
   T *nd = &st[7 * NS], *nv = &st[8 * NS], *na = &st[9 * NS], *npsi = &st[10 * NS], *npsid = &st[11 * NS];
   T *nxis = &st[12 * NS], *nxi = &st[13 * NS], *Ptm = &st[14 * NS];
   T *p = &st[15 * NS], *pn = &st[16 * NS], *Y = &st[17 * NS], *G = &st[18 * NS], *od = &st[19 * NS];
   T *vref = &st[20 * NS], *aref = &st[21 * NS], *ex = &st[22 * NS], *ey = &st[23 * NS], *ez = &st[24 * NS];
+  T *pold = &st[26 * NS], *pext = &st[27 * NS];   // previous step's pressure; extrapolated guess
   T ow_pd = 0, ow_pv = 0, ow_pa = 0, ow_nd = 0, ow_nv = 0, ow_na = 0;
 
   T dt = (T)sc.time_step;
+  T dt_prev = dt;   // dt of the previous step (warm_start 1)
   T t = 0;
   int in_s = it.start;
   auto in_at = [&](int64_t idx) -> T { return (idx >= 0 && idx < input_len) ? (T)input[idx] : (T)0; };
@@ -265,16 +267,27 @@ void bm_item(const orc_bm_scalars &sc, const float *sec, const float *gamma_set,
         for (int i = 0; i < NS; ++i) y2[i] = Y[i] * Y[i];
         const T yy = tree_sum(y2);
         const T tau2 = (T)sc.tol_rel * (T)sc.tol_rel;
-        J = 0;
-        for (;;) {
+        auto resid2 = [&](const T *q) {
           for (int i = 0; i < NS; ++i) {
-            T pm1 = i > 0 ? p[i - 1] : (T)0;
-            T pp1 = i < NS - 1 ? p[i + 1] : (T)0;
+            T pm1 = i > 0 ? q[i - 1] : (T)0;
+            T pp1 = i < NS - 1 ? q[i + 1] : (T)0;
             T off = low[i] * pm1 + pp1;
-            T r = Y[i] - (off + mf[i] * p[i]);
+            T r = Y[i] - (off + mf[i] * q[i]);
             r2[i] = r * r;
           }
-          const T rr = tree_sum(r2);
+          return tree_sum(r2);
+        };
+        if (sc.warm_start == 1 && out_loop == 0) {   // predictor-pass guess
+          const T rho = dt / dt_prev;
+          for (int i = 0; i < NS; ++i) pext[i] = p[i] + rho * (p[i] - pold[i]);
+          const T rr0 = resid2(p), rre = resid2(pext);
+          std::memcpy(pold, p, sizeof(T) * NS);
+          if (rre < rr0) std::memcpy(p, pext, sizeof(T) * NS);
+          dt_prev = dt;
+        }
+        J = 0;
+        for (;;) {
+          const T rr = resid2(p);
           if (rr <= tau2 * yy || J >= sc.max_sweeps) break;
           const int k = std::min(sc.check_every, sc.max_sweeps - J);
           sweep(k);
@@ -455,6 +468,7 @@ int orc_default_scalars(float Fs, orc_bm_scalars *o) {
   o->check_every = 4;
   o->max_sweeps = 1000;
   o->tol_rel = 1e-4f;
+  o->warm_start = 0;
   return 0;
 }
 

@@ -48,6 +48,9 @@ typedef struct {
   int32_t check_every;  /* tolerance mode: sweeps between residual checks */
   int32_t max_sweeps;   /* tolerance mode: sweep cap per pass */
   float tol_rel;        /* tolerance mode: stop when |Y - A p| <= tol_rel |Y| */
+  int32_t warm_start;   /* tolerance mode, predictor pass: 0 = previous pressure; 1 = also try the
+                           linear extrapolation p + (dt/dt_prev)(p - p_prev) and keep the guess
+                           with the smaller residual (ties keep the previous pressure) */
 } orc_bm_scalars;
 
 /* One work item = one reference "time block" (src/cochlea.cu:1148-1176). */

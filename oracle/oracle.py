@@ -26,7 +26,7 @@ class BMScalars(C.Structure):
         "Gme", "a0", "a1", "a2", "sigma_ow", "eta1", "eta2", "dt_min", "dt_max")] + [
         ("j_fast", C.c_int32), ("j_slow", C.c_int32), ("max_passes", C.c_int32),
         ("decision_mode", C.c_int32), ("jacobi_mode", C.c_int32), ("check_every", C.c_int32),
-        ("max_sweeps", C.c_int32), ("tol_rel", C.c_float)]
+        ("max_sweeps", C.c_int32), ("tol_rel", C.c_float), ("warm_start", C.c_int32)]
 
 
 ITEM_DTYPE = np.dtype([("start", np.int32), ("mosi", np.int32), ("mesi", np.int32),

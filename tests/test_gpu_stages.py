@@ -245,12 +245,12 @@ def test_jnd_chain_end_to_end(oracle):
     assert np.all(np.isfinite(ai[serial[sel]]))
 
 
-@pytest.mark.parametrize("tau", [1e-4, 1e-6])
-def test_bm_tolerance_mode_parity(oracle, tau):
+@pytest.mark.parametrize("tau,ws", [(1e-4, 0), (1e-6, 0), (1e-6, 1)])
+def test_bm_tolerance_mode_parity(oracle, tau, ws):
     """Sweep-to-tolerance Jacobi: same residual test and reduction tree on both sides."""
     import torch
     from cochlearmodel_amd import solver
-    _say(f"case: tolerance-mode Jacobi tau={tau}, 2 signals of 600 samples")
+    _say(f"case: tolerance-mode Jacobi tau={tau} warm_start={ws}, 2 signals of 600 samples")
     n = 600
     t = np.arange(n) / FS
     inp = np.zeros(2 * n + 2, np.float32)
@@ -260,17 +260,17 @@ def test_bm_tolerance_mode_parity(oracle, tau):
     items[0] = (0, 0, n, 0, 1e-10, 1e-15, 0, 0)
     items[1] = (n, n, 2 * n, n, 1e-10, 1e-15, 0, 0)
     sc = solver.default_scalars(FS)
-    sc.jacobi_mode, sc.tol_rel, sc.check_every, sc.max_sweeps = 1, tau, 4, 1000
+    sc.jacobi_mode, sc.tol_rel, sc.check_every, sc.max_sweeps, sc.warm_start = 1, tau, 4, 1000, ws
     bb = solver.BatchedBM(np.full(256, 0.5, np.float32))
     out = bb.run(sc, torch.from_numpy(inp).cuda(), items, 2 * n)
     v = out["speeds"].cpu().numpy()
     st = out["stats"].cpu().numpy()
     so = oracle.default_scalars(FS)
-    so.jacobi_mode, so.tol_rel, so.check_every, so.max_sweeps = 1, tau, 4, 1000
+    so.jacobi_mode, so.tol_rel, so.check_every, so.max_sweeps, so.warm_start = 1, tau, 4, 1000, ws
     it = np.zeros(2, oracle.ITEM_DTYPE)
     for f in ("start", "mosi", "mesi", "write_start", "tol", "m1tol", "set"):
         it[f] = items[f]
     o = oracle.bm_run(so, oracle.section_params(), np.full(256, 0.5, np.float32), it, inp, 2 * n, "f32", threads=2)
     _say(f"rel L2 {_rel(v, o['speeds']):.3e}, sweeps gpu {st[:, 2]} oracle {o['sweeps']}")
-    assert _rel(v, o["speeds"]) <= 1e-4
+    np.testing.assert_array_equal(v, o["speeds"])
     np.testing.assert_array_equal(st[:, 2], o["sweeps"])

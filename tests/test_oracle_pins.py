@@ -320,3 +320,35 @@ def test_input_generation_closed_form(oracle):
     for p in range(2):
         exp = dA[p] * np.cos(2 * np.float32(np.pi) * fr[p] * (n - head) / Fs) / 7.0 + Wn[p] * WN / 3.0
         np.testing.assert_allclose(out[p * L:(p + 1) * L], exp, rtol=1e-6, atol=1e-9)
+
+
+# ------------------------------------------------------ tolerance mode ---
+def _tol_run(oracle, sig, mode, tau=1e-6, ws=0):
+    sc = oracle.default_scalars(44100.0)
+    sc.jacobi_mode, sc.tol_rel, sc.check_every, sc.max_sweeps, sc.warm_start = mode, tau, 4, 1000, ws
+    n = len(sig)
+    inp = np.zeros(n + 4, np.float32)
+    inp[:n] = sig
+    it = oracle.make_items([0], [0], [n], [0], 1e-10, 1e-15)
+    return oracle.bm_run(sc, oracle.section_params(), np.full(256, 0.5, np.float32), it, inp, n, "f32", threads=1)
+
+
+def test_tolerance_mode_converges_to_exact_solve(oracle):
+    """Jacobi to a relative residual tau approaches the exact-pressure method
+    (O2) as tau falls, and the extrapolated warm start keeps that accuracy
+    while cutting sweeps: a wrong sign or step ratio in the extrapolation makes
+    it worse than the previous pressure, the residual gate then keeps the
+    previous pressure and the sweep count does not fall."""
+    sig = _tone(1000, 60, 300)
+    o2 = _tol_run(oracle, sig, 2)["speeds"][40:]
+    rel = lambda a: float(np.linalg.norm(a[40:] - o2) / np.linalg.norm(o2))
+    e = {}
+    for tau in (1e-4, 1e-6):
+        o = _tol_run(oracle, sig, 1, tau)
+        e[tau] = rel(o["speeds"])
+    assert e[1e-6] < 0.2 * e[1e-4] and e[1e-6] < 1.5e-3
+    plain = _tol_run(oracle, sig, 1, 1e-6, ws=0)
+    ext = _tol_run(oracle, sig, 1, 1e-6, ws=1)
+    assert rel(ext["speeds"]) < 1.5e-3
+    assert ext["sweeps"][0] < 0.6 * plain["sweeps"][0], (ext["sweeps"], plain["sweeps"])
+    assert abs(int(ext["steps"][0]) - int(plain["steps"][0])) <= 0.01 * plain["steps"][0]
