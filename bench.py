@@ -36,94 +36,10 @@ def flops_per_launch(steps: int, passes: int, sweeps: int) -> float:
     return SECTIONS * (142.0 * steps + 49.0 * (passes - 2 * steps) + 4.0 * sweeps)
 
 
-class C2Workload:
-    def __init__(self, n_signals: int = 256, duration: float = 0.2):
-        import torch
-        import ctypes as C
-        from cochlearmodel_amd import _lib, solver, stages, workloads
-        self.torch, self.C = torch, C
-        self.L = _lib.lib()
-        self._lib = _lib
-        f, lv = workloads.c2_grid()
-        f, lv = f[:n_signals], lv[:n_signals]
-        self.n = n_signals
-        self.interval = int(round(FS * duration))                  # JND_Interval_Nodes_Full
-        self.block = int(math.ceil(np.float32(0.02) * np.float32(FS)))
-        self.DF = self.interval // self.block
-        self.TB = self.n * self.DF
-        self.ov = 220
-        self.bufP1 = self.ov + self.block * (self.TB + 1)
-        self.rows = self.n * self.interval
-        head = int(round(FS * 0.014))
-        # one profile per (frequency, level), no JND references (Sim_Type=3)
-        prof = np.zeros(self.n, _lib.PROFILE_DTYPE)
-        prof["dA"] = 10.0 * 10.0 ** (lv / 20.0) * 2e-5
-        prof["recipdA"] = 1.0 / prof["dA"]
-        prof["dBSPLSignal"] = lv
-        prof["frequency"] = f
-        prof["referenceIndex"] = -1
-        prof["calculateIndexPosition"] = np.arange(self.n)
-        self.profiles = prof
-        rng = np.random.default_rng(workloads.MASTER_SEED + 2)
-        WN = rng.standard_normal(self.interval).astype(np.float32)
-        e1 = int((duration - 0.002) * FS)
-        n4 = np.zeros(4, np.float64)
-        _lib.check(self.L.cm_input_normalization(_lib.ptr(WN), len(WN), None, 0, 0, 0, 0, 0.16, 0.0, 0, e1, 0, e1,
-                                                 C.c_float(FS), _lib.ptr(n4)), "normalization")
-        self.norms = n4
-        self.head = head
-        dev = "cuda"
-        self.d_prof = torch.from_numpy(prof.view(np.uint8).copy()).to(dev)
-        self.d_WN = torch.from_numpy(WN).to(dev)
-        self.d_input = torch.zeros(self.bufP1 + 2, dtype=torch.float32, device=dev)
-        items = solver.block_table(self.TB, self.bufP1, self.ov, self.DF)
-        items["tol"], items["m1tol"] = 1e-10, 1e-15          # setupToleranceProfile, absolute errors
-        self.items = items
-        self.sc = solver.default_scalars(FS)
-        self.bm = solver.BatchedBM(np.full(SECTIONS, 0.5, np.float32))
-        self.d_items = torch.from_numpy(items.view(np.uint8).copy()).to(dev)
-        self.d_speeds = torch.zeros((self.bufP1, SECTIONS), dtype=torch.float32, device=dev)
-        self.d_stats = torch.zeros(self.TB * 3, dtype=torch.int64, device=dev)
-        self.ihc = stages.BatchedIHC(FS)
-        self.fir_ivs, _ = stages.reference_intervals(self.rows, self.block, self.TB, self.DF)
-        self.d_lambda = torch.empty((3, self.rows, SECTIONS), dtype=torch.float32, device=dev)
-        self.d_scratch = torch.empty((2, self.rows, SECTIONS), dtype=torch.float32, device=dev)
-        self.stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
-
-    def run_bm(self, sc, timed: bool = True):
-        C, L, lib = self.C, self.L, self._lib
-        L.cm_set_stream(self.stream)
-        lib.check(L.cm_bm_run(C.byref(sc), lib.ptr(self.d_input), self.d_input.numel(), lib.ptr(self.bm.sec),
-                              lib.ptr(self.bm.gamma), lib.ptr(self.d_items), self.TB, lib.ptr(self.d_speeds),
-                              self.bufP1, None, None, None, None, None, None, None, None, lib.ptr(self.d_stats), 1, 0,
-                              1 if timed else 0), "bm")
-        return L.cm_last_kernel_ms() if timed else None
-
-    def step(self, sc, timed_bm: bool = True):
-        torch, C, L, lib = self.torch, self.C, self.L, self._lib
-        L.cm_set_stream(self.stream)
-        t = {}
-        e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
-        e0.record()
-        lib.check(L.cm_generate_input_run(lib.ptr(self.d_prof), lib.ptr(self.d_WN), None, lib.ptr(self.d_input),
-                                          self.norms[0], self.norms[1], 0, self.norms[2], self.norms[3], 0, self.n,
-                                          self.interval, self.head, C.c_float(FS)), "input")
-        e1.record()
-        ms = self.run_bm(sc, timed_bm)
-        if timed_bm:
-            t["bm_ms"] = ms
-        e2.record()
-        self.ihc.run(self.d_speeds, self.rows, self.fir_ivs, d_lambda=self.d_lambda, d_scratch=self.d_scratch)
-        e3 = torch.cuda.Event(enable_timing=True)
-        e3.record()
-        e3.synchronize()
-        t["input_ms"] = e0.elapsed_time(e1)
-        t["ihc_ms"] = e2.elapsed_time(e3)
-        return t
-
-    def stats(self):
-        s = self.d_stats.view(self.TB, 3).cpu().numpy().astype(np.int64)
-        return int(s[:, 0].sum()), int(s[:, 1].sum()), int(s[:, 2].sum())
+def c2_workload(n_signals: int = 256):
+    from cochlearmodel_amd import batch, workloads
+    f, lv = workloads.c2_grid()
+    return batch.ToneBatch(f[:n_signals], lv[:n_signals], 0.2)
 
 
 TOL = 1e-6   # relative residual of the tolerance-mode Jacobi
@@ -144,8 +60,7 @@ VARIANTS = ("reference_20_2", "plain_tol", "extrapolated_tol")
 
 
 def sample_blocks(work, n_blocks):
-    stride = max(1, work.TB // n_blocks)
-    return np.arange(0, work.TB, stride)[:n_blocks]
+    return work.sample_blocks(n_blocks)
 
 
 def oracle_items(work, sel):
@@ -157,8 +72,7 @@ def oracle_items(work, sel):
 
 
 def block_rows(work, sel):
-    ws, me = work.items["write_start"][sel], work.items["mesi"][sel]
-    return np.concatenate([np.arange(a, b) for a, b in zip(ws, me)])
+    return work.block_rows(sel)
 
 
 def exact_reference(work, n_blocks, threads):
@@ -174,7 +88,7 @@ def exact_reference(work, n_blocks, threads):
     return rows, o["speeds"][rows]
 
 
-def cpu_baseline(work: C2Workload, n_blocks: int, threads: int, sc_gpu):
+def cpu_baseline(work, n_blocks: int, threads: int, sc_gpu):
     """The CPU oracle (FP32 port of kernel 17) on every (TB/n_blocks)-th block."""
     from oracle import oracle as O
     inp = work.d_input.cpu().numpy()
@@ -213,7 +127,7 @@ def main():
         sys.exit(2)
     import torch
     torch.cuda.set_device(0)
-    work = C2Workload(n_signals=args.signals)
+    work = c2_workload(args.signals)
     sc = variant_scalars(args.variant)
     print(f"# workload ready: {work.n} signals, {work.TB} time blocks, variant {args.variant}", file=sys.stderr,
           flush=True)

@@ -123,7 +123,13 @@ _SIGS = {
     "cm_colsum": (C.c_int, [vp, vp, C.c_int, C.c_int]),
     "cm_mse_grad": (C.c_int, [vp, vp, vp, vp, vp, C.c_int, C.c_int]),
     "cm_adam": (C.c_int, [vp, vp, vp, vp, vp, i64, f32, f32, f32, f32, C.c_int, f32]),
+    "cm_ws_load": (C.c_int, [vp, vp, vp, vp, vp, vp, f32]),
+    "cm_ws_set_capture": (C.c_int, [vp, i64, C.c_int, C.c_int]),
+    "cm_ws_targets": (C.c_int, [vp, vp, i64, f32, vp]),
+    "cm_ws_release": (C.c_int, []),
 }
+WS_RECORD = 1544      # floats per captured training record
+WS_REC_Y, WS_REC_PLIN, WS_REC_OUT = 772, 1028, 1288
 
 
 class CochleaError(RuntimeError):

@@ -1,0 +1,129 @@
+"""A device-resident batch of tone profiles in the reference's profile layout
+(Sim_Type 3: one interval per tone, Decouple_Filter = interval / 20 ms block,
+src/cochlea.cu:1148-1176), driven through the C-ABI: input synthesis
+(generateInputFromProfile semantics), the BM solve and IHC/lambda.
+
+torch tensors hold device memory and the stream; all compute is the library's.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _lib, solver, stages, workloads
+
+FS = workloads.FS
+SECTIONS = 256
+
+
+class ToneBatch:
+    def __init__(self, freqs, levels, duration: float = 0.2, with_lambda: bool = True, wn_seed_offset: int = 2):
+        import torch
+        self.torch = torch
+        self.L = _lib.lib()
+        freqs = np.asarray(freqs, np.float64)
+        levels = np.asarray(levels, np.float64)
+        self.n = len(freqs)
+        self.freqs, self.levels = freqs, levels
+        self.interval = int(round(FS * duration))                  # JND_Interval_Nodes_Full
+        self.block = int(math.ceil(np.float32(0.02) * np.float32(FS)))
+        self.DF = self.interval // self.block
+        self.TB = self.n * self.DF
+        self.ov = 220
+        self.bufP1 = self.ov + self.block * (self.TB + 1)
+        self.rows = self.n * self.interval
+        self.head = int(round(FS * 0.014))
+        prof = np.zeros(self.n, _lib.PROFILE_DTYPE)
+        prof["dA"] = 10.0 * 10.0 ** (levels / 20.0) * 2e-5
+        prof["recipdA"] = 1.0 / prof["dA"]
+        prof["dBSPLSignal"] = levels
+        prof["frequency"] = freqs
+        prof["referenceIndex"] = -1
+        prof["calculateIndexPosition"] = np.arange(self.n)
+        self.profiles = prof
+        rng = np.random.default_rng(workloads.MASTER_SEED + wn_seed_offset)
+        WN = rng.standard_normal(self.interval).astype(np.float32)
+        e1 = int((duration - 0.002) * FS)
+        n4 = np.zeros(4, np.float64)
+        _lib.check(self.L.cm_input_normalization(_lib.ptr(WN), len(WN), None, 0, 0, 0, 0, 0.16, 0.0, 0, e1, 0, e1,
+                                                 C.c_float(FS), _lib.ptr(n4)), "normalization")
+        self.norms = n4
+        dev = "cuda"
+        self.d_prof = torch.from_numpy(prof.view(np.uint8).copy()).to(dev)
+        self.d_WN = torch.from_numpy(WN).to(dev)
+        self.d_input = torch.zeros(self.bufP1 + 2, dtype=torch.float32, device=dev)
+        items = solver.block_table(self.TB, self.bufP1, self.ov, self.DF)
+        items["tol"], items["m1tol"] = 1e-10, 1e-15          # setupToleranceProfile, absolute errors
+        self.items = items
+        self.bm = solver.BatchedBM(np.full(SECTIONS, 0.5, np.float32))
+        self.d_items = torch.from_numpy(items.view(np.uint8).copy()).to(dev)
+        self.d_speeds = torch.zeros((self.bufP1, SECTIONS), dtype=torch.float32, device=dev)
+        self.d_stats = torch.zeros(self.TB * 3, dtype=torch.int64, device=dev)
+        self.with_lambda = with_lambda
+        if with_lambda:
+            self.ihc = stages.BatchedIHC(FS)
+            self.fir_ivs, _ = stages.reference_intervals(self.rows, self.block, self.TB, self.DF)
+            self.d_lambda = torch.empty((3, self.rows, SECTIONS), dtype=torch.float32, device=dev)
+            self.d_scratch = torch.empty((2, self.rows, SECTIONS), dtype=torch.float32, device=dev)
+        self.stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def generate_input(self):
+        L, lib = self.L, _lib
+        L.cm_set_stream(self.stream)
+        lib.check(L.cm_generate_input_run(lib.ptr(self.d_prof), lib.ptr(self.d_WN), None, lib.ptr(self.d_input),
+                                          self.norms[0], self.norms[1], 0, self.norms[2], self.norms[3], 0, self.n,
+                                          self.interval, self.head, C.c_float(FS)), "input")
+
+    def run_bm(self, sc, timed: bool = True):
+        L, lib = self.L, _lib
+        L.cm_set_stream(self.stream)
+        lib.check(L.cm_bm_run(C.byref(sc), lib.ptr(self.d_input), self.d_input.numel(), lib.ptr(self.bm.sec),
+                              lib.ptr(self.bm.gamma), lib.ptr(self.d_items), self.TB, lib.ptr(self.d_speeds),
+                              self.bufP1, None, None, None, None, None, None, None, None, lib.ptr(self.d_stats), 1, 0,
+                              1 if timed else 0), "bm")
+        return L.cm_last_kernel_ms() if timed else None
+
+    def step(self, sc, timed_bm: bool = True):
+        """Input synthesis + BM + IHC/lambda; per-stage times in ms."""
+        torch = self.torch
+        t = {}
+        e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        e0.record()
+        self.generate_input()
+        e1.record()
+        ms = self.run_bm(sc, timed_bm)
+        if timed_bm:
+            t["bm_ms"] = ms
+        e2.record()
+        if self.with_lambda:
+            self.ihc.run(self.d_speeds, self.rows, self.fir_ivs, d_lambda=self.d_lambda, d_scratch=self.d_scratch)
+        e3 = torch.cuda.Event(enable_timing=True)
+        e3.record()
+        e3.synchronize()
+        t["input_ms"] = e0.elapsed_time(e1)
+        t["ihc_ms"] = e2.elapsed_time(e3)
+        return t
+
+    def stats(self):
+        s = self.d_stats.view(self.TB, 3).cpu().numpy().astype(np.int64)
+        return int(s[:, 0].sum()), int(s[:, 1].sum()), int(s[:, 2].sum())
+
+    # -- helpers for sampled CPU checks ----------------------------------
+    def sample_blocks(self, n_blocks):
+        stride = max(1, self.TB // n_blocks)
+        return np.arange(0, self.TB, stride)[:n_blocks]
+
+    def block_rows(self, sel):
+        ws, me = self.items["write_start"][sel], self.items["mesi"][sel]
+        return np.concatenate([np.arange(a, b) for a, b in zip(ws, me)])
+
+
+def random_tones(n: int, seed: int, f_lo: float = 250.0, f_hi: float = 8000.0, db_lo: float = 20.0,
+                 db_hi: float = 90.0):
+    """Tones drawn like the 256-tone grid: frequency log-uniform, level uniform."""
+    rng = np.random.default_rng(seed)
+    f = np.exp(rng.uniform(np.log(f_lo), np.log(f_hi), n))
+    db = rng.uniform(db_lo, db_hi, n)
+    return f, db

@@ -119,27 +119,54 @@ __device__ __forceinline__ void jacobi_sweeps(const float (&Y)[4], const float (
   }
 }
 
-// Squared 2-norm of the residual Y - A p (kernel convention, fixed tree).
-__device__ __forceinline__ float resid2(const float (&Y)[4], const float (&mf)[4], float low23, const float (&p)[4]) {
+
+// Residual Y - A p of the kernel convention, per lane, and its squared 2-norm
+// with the fixed tree.
+__device__ __forceinline__ void resid_vec(const float (&Y)[4], const float (&mf)[4], float low23, const float (&p)[4],
+                                          float (&r)[4]) {
   const float left = wave_shr1(p[3]);
   const float right = wave_shl1(p[0]);
-  const float r0 = Y[0] - ((left + p[1]) + mf[0] * p[0]);
-  const float r1 = Y[1] - ((p[0] + p[2]) + mf[1] * p[1]);
-  const float r2 = Y[2] - ((low23 * p[1] + p[3]) + mf[2] * p[2]);
-  const float r3 = Y[3] - ((low23 * p[2] + right) + mf[3] * p[3]);
-  return tree_sum(r0 * r0, r1 * r1, r2 * r2, r3 * r3);
+  r[0] = Y[0] - ((left + p[1]) + mf[0] * p[0]);
+  r[1] = Y[1] - ((p[0] + p[2]) + mf[1] * p[1]);
+  r[2] = Y[2] - ((low23 * p[1] + p[3]) + mf[2] * p[2]);
+  r[3] = Y[3] - ((low23 * p[2] + right) + mf[3] * p[3]);
+}
+__device__ __forceinline__ float resid2(const float (&Y)[4], const float (&mf)[4], float low23, const float (&p)[4]) {
+  float r[4];
+  resid_vec(Y, mf, low23, p, r);
+  return tree_sum(r[0] * r[0], r[1] * r[1], r[2] * r[2], r[3] * r[3]);
 }
 
-// One time block, include/cochlea.cuh:13620-14232.  TOL selects the
-// sweep-to-tolerance Jacobi; otherwise the reference's fixed 20/2 sweeps.
-// WS = 1 adds the extrapolated predictor-pass guess (tolerance mode only).
-template <bool TOL, int WS>
-__device__ void run_item(const BmLaunch &a, int item_index, int lane, float4 *pold_lds) {
-  const cm_bm_item it = a.items[item_index];
-  const cm_bm_scalars &sc = a.sc;
-  const int base = 4 * lane;
-
+// ------------------------------------------------- one time block -------
+// Per-lane constants of an item: section parameters of sections 4l..4l+3.
+struct ItemConst {
   float rsm[4], rmass[4], gam[4], R[4], S[4], Q[4], Sohc[4], Stm[4], Rtm[4], mf[4];
+  float low23;   // lower-coupling multiplier of slots 2, 3 (0 for sections 254, 255)
+};
+
+// Scalars shared by all lanes (uniform).
+struct ItemScal {
+  float w_ohc, alpha_r, Ts, al, Gme, a1, sigma_ow, eta1, eta2, dt_min, dt_max, tso, mtt, mmspt;
+  float al_over_as, dxdx, dxa0, neg_a1, neg_a2, neg_w, neg_as, span;
+};
+
+// Integration state of an item (include/cochlea.cuh:13620-13700 names in comments).
+struct ItemState {
+  float pd[4], pv[4], pa[4], ppsi[4], ppsid[4], pxis[4], pxi[4];    // committed
+  float nd[4], nv[4], na[4], npsi[4], npsid[4], nxis[4], nxi[4], Ptm[4];
+  float p[4];                                                          // resident pressure
+  float ow_pd, ow_pv, ow_pa, ow_nd, ow_nv, ow_na;                      // oval window
+  float dt, t, sx0, sx1;
+  int in_s;
+  float c_fail, c_dt, c_steps, c_passes;                               // sx[11], sx[12], sx[13], sx[15]
+  int64_t n_steps, n_passes, n_sweeps;
+  int32_t fb;
+  float ctb, clb, cjb;
+};
+
+template <bool TOL>
+__device__ __forceinline__ void load_consts(const BmLaunch &a, const cm_bm_item &it, int lane, ItemConst &k) {
+  const int base = 4 * lane;
   {
     const float4 m4 = *reinterpret_cast<const float4 *>(a.sec + CM_SEC_MASS * CM_SECTIONS + base);
     const float4 r4 = *reinterpret_cast<const float4 *>(a.sec + CM_SEC_RSM * CM_SECTIONS + base);
@@ -151,242 +178,312 @@ __device__ void run_item(const BmLaunch &a, int item_index, int lane, float4 *po
     const float4 U4 = *reinterpret_cast<const float4 *>(a.sec + CM_SEC_RTM * CM_SECTIONS + base);
     const float4 G4 = *reinterpret_cast<const float4 *>(a.gamma + (int64_t)it.set * CM_SECTIONS + base);
     const float mm[4] = {m4.x, m4.y, m4.z, m4.w};
-    rsm[0] = r4.x; rsm[1] = r4.y; rsm[2] = r4.z; rsm[3] = r4.w;
-    R[0] = R4.x; R[1] = R4.y; R[2] = R4.z; R[3] = R4.w;
-    S[0] = S4.x; S[1] = S4.y; S[2] = S4.z; S[3] = S4.w;
-    Q[0] = Q4.x; Q[1] = Q4.y; Q[2] = Q4.z; Q[3] = Q4.w;
-    Sohc[0] = O4.x; Sohc[1] = O4.y; Sohc[2] = O4.z; Sohc[3] = O4.w;
-    Stm[0] = T4.x; Stm[1] = T4.y; Stm[2] = T4.z; Stm[3] = T4.w;
-    Rtm[0] = U4.x; Rtm[1] = U4.y; Rtm[2] = U4.z; Rtm[3] = U4.w;
-    gam[0] = G4.x; gam[1] = G4.y; gam[2] = G4.z; gam[3] = G4.w;
-    UNROLL for (int j = 0; j < 4; ++j) rmass[j] = 1.0f / mm[j];
+    k.rsm[0] = r4.x; k.rsm[1] = r4.y; k.rsm[2] = r4.z; k.rsm[3] = r4.w;
+    k.R[0] = R4.x; k.R[1] = R4.y; k.R[2] = R4.z; k.R[3] = R4.w;
+    k.S[0] = S4.x; k.S[1] = S4.y; k.S[2] = S4.z; k.S[3] = S4.w;
+    k.Q[0] = Q4.x; k.Q[1] = Q4.y; k.Q[2] = Q4.z; k.Q[3] = Q4.w;
+    k.Sohc[0] = O4.x; k.Sohc[1] = O4.y; k.Sohc[2] = O4.z; k.Sohc[3] = O4.w;
+    k.Stm[0] = T4.x; k.Stm[1] = T4.y; k.Stm[2] = T4.z; k.Stm[3] = T4.w;
+    k.Rtm[0] = U4.x; k.Rtm[1] = U4.y; k.Rtm[2] = U4.z; k.Rtm[3] = U4.w;
+    k.gam[0] = G4.x; k.gam[1] = G4.y; k.gam[2] = G4.z; k.gam[3] = G4.w;
+    UNROLL for (int j = 0; j < 4; ++j) k.rmass[j] = 1.0f / mm[j];
     if (TOL) {
       const float4 M4 = *reinterpret_cast<const float4 *>(a.sec + CM_SEC_M * CM_SECTIONS + base);
-      mf[0] = M4.x; mf[1] = M4.y; mf[2] = M4.z; mf[3] = M4.w;
+      k.mf[0] = M4.x; k.mf[1] = M4.y; k.mf[2] = M4.z; k.mf[3] = M4.w;
     } else {
-      UNROLL for (int j = 0; j < 4; ++j) mf[j] = 0.0f;
+      UNROLL for (int j = 0; j < 4; ++j) k.mf[j] = 0.0f;
     }
   }
-  // low multiplier: 0 for sections 254 and 255 (lane 63, slots 2 and 3)
-  const float low23 = lane == 63 ? 0.0f : 1.0f;
+  k.low23 = lane == 63 ? 0.0f : 1.0f;
+}
 
-  const float w_ohc = sc.w_ohc, dx = sc.delta_x, alpha_r = sc.alpha_r, Ts = sc.Ts;
-  const float al = sc.alpha_l, as = sc.alpha_s, Gme = sc.Gme, a1 = sc.a1, a2 = sc.a2;
-  const float sigma_ow = sc.sigma_ow, eta1 = sc.eta1, eta2 = sc.eta2;
-  const float dt_min = sc.dt_min, dt_max = sc.dt_max, tso = sc.time_step_out;
-  const float mtt = it.tol, mmspt = it.m1tol;
-  const float al_over_as = al / as;
-  const float dxdx = dx * dx;
-  const float dxa0 = dx * sc.a0;
-  const float neg_a1 = -a1, neg_a2 = -a2, neg_w = -w_ohc, neg_as = -as;
+template <bool TOL>
+__device__ __forceinline__ void item_setup(const BmLaunch &a, const cm_bm_item &it, int lane, ItemConst &k,
+                                           ItemScal &c, ItemState &s) {
+  const cm_bm_scalars &sc = a.sc;
+  load_consts<TOL>(a, it, lane, k);
 
-  float pd[4] = {0, 0, 0, 0}, pv[4] = {0, 0, 0, 0}, pa[4] = {0, 0, 0, 0}, ppsi[4] = {0, 0, 0, 0};
-  float ppsid[4] = {0, 0, 0, 0}, pxis[4] = {0, 0, 0, 0}, pxi[4] = {0, 0, 0, 0};
-  float nd[4] = {0, 0, 0, 0}, nv[4] = {0, 0, 0, 0}, na[4] = {0, 0, 0, 0}, npsi[4] = {0, 0, 0, 0};
-  float npsid[4] = {0, 0, 0, 0}, nxis[4] = {0, 0, 0, 0}, nxi[4] = {0, 0, 0, 0}, Ptm[4] = {0, 0, 0, 0};
-  float p[4] = {0, 0, 0, 0};
-  // previous step's pressure (WS == 1) lives in LDS: it is touched once per step
-  if (WS == 1) pold_lds[lane] = make_float4(0.f, 0.f, 0.f, 0.f);
-  float ow_pd = 0, ow_pv = 0, ow_pa = 0, ow_nd = 0, ow_nv = 0, ow_na = 0;
+  c.w_ohc = sc.w_ohc; c.alpha_r = sc.alpha_r; c.Ts = sc.Ts;
+  c.al = sc.alpha_l; c.Gme = sc.Gme; c.a1 = sc.a1;
+  c.sigma_ow = sc.sigma_ow; c.eta1 = sc.eta1; c.eta2 = sc.eta2;
+  c.dt_min = sc.dt_min; c.dt_max = sc.dt_max; c.tso = sc.time_step_out;
+  c.mtt = it.tol; c.mmspt = it.m1tol;
+  c.al_over_as = sc.alpha_l / sc.alpha_s;
+  c.dxdx = sc.delta_x * sc.delta_x;
+  c.dxa0 = sc.delta_x * sc.a0;
+  c.neg_a1 = -sc.a1; c.neg_a2 = -sc.a2; c.neg_w = -sc.w_ohc; c.neg_as = -sc.alpha_s;
+  c.span = (float)(it.mesi - it.mosi);
 
-  float dt = sc.time_step;
-  // dt of the previous step (WS == 1), kept in the spare LDS slot after the pressures
-  float *dt_prev_lds = reinterpret_cast<float *>(pold_lds + 64);
-  if (WS == 1 && lane == 0) *dt_prev_lds = dt;
-  float t = 0.0f;
-  int in_s = it.start;
-  float sx0 = in_at(a, in_s), sx1 = in_at(a, (int64_t)in_s + 1);
-  float c_fail = 0.f, c_dt = 0.f, c_steps = 0.f, c_passes = 0.f;
-  int64_t n_steps = 0, n_passes = 0, n_sweeps = 0;
-  int32_t fb = 0;
-  float ctb = 0.f, clb = 0.f, cjb = 0.f;
-  const float span = (float)(it.mesi - it.mosi);
+  UNROLL for (int j = 0; j < 4; ++j) {
+    s.pd[j] = 0; s.pv[j] = 0; s.pa[j] = 0; s.ppsi[j] = 0; s.ppsid[j] = 0; s.pxis[j] = 0; s.pxi[j] = 0;
+    s.nd[j] = 0; s.nv[j] = 0; s.na[j] = 0; s.npsi[j] = 0; s.npsid[j] = 0; s.nxis[j] = 0; s.nxi[j] = 0;
+    s.Ptm[j] = 0; s.p[j] = 0;
+  }
+  s.ow_pd = 0; s.ow_pv = 0; s.ow_pa = 0; s.ow_nd = 0; s.ow_nv = 0; s.ow_na = 0;
+  s.dt = sc.time_step;
+  s.t = 0.0f;
+  s.in_s = it.start;
+  s.sx0 = in_at(a, s.in_s);
+  s.sx1 = in_at(a, (int64_t)s.in_s + 1);
+  s.c_fail = 0.f; s.c_dt = 0.f; s.c_steps = 0.f; s.c_passes = 0.f;
+  s.n_steps = 0; s.n_passes = 0; s.n_sweeps = 0;
+  s.fb = 0;
+  s.ctb = 0.f; s.clb = 0.f; s.cjb = 0.f;
+}
 
-  while (in_s < it.mesi) {
+// Predictor (Euler) or corrector (trapezoid) update, :13770-13820.
+__device__ __forceinline__ void pass_begin(ItemState &s, bool predictor) {
+  const float dt = s.dt;
+  if (predictor) {
+    UNROLL for (int j = 0; j < 4; ++j) {
+      s.nxi[j] = s.pxis[j] * dt + s.pxi[j];
+      s.nv[j] = s.pa[j] * dt + s.pv[j];
+      s.nd[j] = ((s.pv[j] + s.nv[j]) * 0.5f) * dt + s.pd[j];
+      s.npsi[j] = s.ppsid[j] * dt + s.ppsi[j];
+    }
+    s.ow_nd = s.ow_pv * dt + s.ow_pd;
+    s.ow_nv = s.ow_pa * dt + s.ow_pv;
+  } else {
+    UNROLL for (int j = 0; j < 4; ++j) {
+      s.nv[j] = ((s.pa[j] + s.na[j]) * 0.5f) * dt + s.pv[j];
+      s.nd[j] = ((s.pv[j] + s.nv[j]) * 0.5f) * dt + s.pd[j];
+      s.nxi[j] = ((s.pxis[j] + s.nxis[j]) * 0.5f) * dt + s.pxi[j];
+      s.npsi[j] = ((s.ppsid[j] + s.npsid[j]) * 0.5f) * dt + s.ppsi[j];
+    }
+    s.ow_nd = ((s.ow_pv + s.ow_nv) * 0.5f) * dt + s.ow_pd;
+    s.ow_nv = ((s.ow_pa + s.ow_na) * 0.5f) * dt + s.ow_pv;
+  }
+}
+
+// TM/OHC block (passes < 2, :13839-13898), G (:13905-13909) and the right-hand
+// side with its boundaries (:13925-13954).  Returns the forcing s(t).
+__device__ __forceinline__ float pass_rhs(const ItemConst &k, const ItemScal &c, ItemState &s, bool aux_pass, int lane,
+                                          float (&od)[4], float (&G)[4], float (&Y)[4]) {
+  UNROLL for (int j = 0; j < 4; ++j) {
+    od[j] = s.nxi[j] - s.nd[j];
+    const float ohc_p = (k.gam[j] * k.Sohc[j]) * od[j];
+    if (aux_pass) {
+      float aux = cm_tanhf(c.al_over_as * s.npsi[j]);
+      const float dL = c.neg_as * aux;
+      aux = aux * aux - 1.0f;
+      aux = c.al * aux;
+      float n = (c.eta2 * s.nv[j] + c.eta1 * od[j]) - c.w_ohc * s.npsi[j];
+      n = k.Stm[j] * dL + (n * k.Rtm[j]) * aux;
+      const float ptmp = k.Stm[j] * s.nxi[j] + ohc_p;
+      n = ptmp - k.gam[j] * n;
+      float den = (k.gam[j] * aux) * c.eta2 - 1.0f;
+      den = k.Rtm[j] * den;
+      s.nxis[j] = n / den;
+      s.Ptm[j] = k.Rtm[j] * s.nxis[j] + k.Stm[j] * s.nxi[j];
+    }
+    float g = s.nv[j] * s.nv[j];
+    g = k.R[j] * (c.alpha_r * g + 1.0f);
+    g = g * s.nv[j] + k.S[j] * s.nd[j];
+    G[j] = -g - s.Ptm[j];
+    Y[j] = (c.dxdx * G[j]) * k.Q[j];
+  }
+  const float delta = s.t / c.Ts;
+  const float sf = s.sx1 * delta + s.sx0 * (1.0f - delta);
+  const float Y0 = c.dxa0 * (((c.neg_a2 * s.ow_nv) - c.a1 * s.ow_nd) + c.Gme * sf);
+  if (lane == 0) Y[0] = Y0;
+  if (lane == 63) Y[3] = 0.0f;
+  return sf;
+}
+
+// Jacobi until |Y - A p|_2 <= tol |Y|_2 (checked every check_every sweeps,
+// capped at max_sweeps); returns the sweeps done.
+__device__ __forceinline__ int jacobi_tol(const cm_bm_scalars &sc, const ItemConst &k, const float (&Y)[4],
+                                          float yy, float (&p)[4]) {
+  const float tau2 = sc.tol_rel * sc.tol_rel;
+  int J = 0;
+  for (;;) {
+    const float rr = resid2(Y, k.mf, k.low23, p);
+    if (rr <= tau2 * yy || J >= sc.max_sweeps) break;
+    const int kk = sc.check_every < sc.max_sweeps - J ? sc.check_every : sc.max_sweeps - J;
+    jacobi_sweeps(Y, k.rsm, k.low23, p, kk);
+    J += kk;
+  }
+  return J;
+}
+
+// Acceleration, oval-window acceleration and psi derivative, :13989-14013.
+__device__ __forceinline__ void pass_finish(const ItemConst &k, const ItemScal &c, ItemState &s, const float (&od)[4],
+                                            const float (&G)[4], float sf) {
+  UNROLL for (int j = 0; j < 4; ++j) s.na[j] = (s.p[j] + G[j]) * k.rmass[j];
+  {
+    const float p0 = readlane(s.p[0], 0);
+    const float NN1 = c.neg_a2 * s.ow_nv;
+    const float NN2 = c.Gme * sf + p0;
+    const float NN3 = c.neg_a1 * s.ow_nd;
+    s.ow_na = ((NN1 + NN2) + NN3) / c.sigma_ow;
+  }
+  UNROLL for (int j = 0; j < 4; ++j)
+    s.npsid[j] = (c.eta1 * od[j] + c.eta2 * (s.nxis[j] - s.nv[j])) + c.neg_w * s.npsi[j];
+}
+
+// Step control (:14021-14088) and the pass counter; Q1 lane-0 semantics.
+__device__ __forceinline__ void pass_control(const cm_bm_scalars &sc, const ItemScal &c, ItemState &s,
+                                             const float (&vref)[4], const float (&aref)[4], int &out_loop,
+                                             int &reset, int &another) {
+  if (reset == 0) {
+    float ex[4], ey[4], ez[4];
+    UNROLL for (int j = 0; j < 4; ++j) {
+      ex[j] = __builtin_fabsf(aref[j] - s.na[j]);
+      ey[j] = __builtin_fabsf(s.nv[j] - vref[j]);
+      ez[j] = ey[j] - fmaxf(c.mtt, s.nv[j] * s.dt);
+    }
+    float emax = tree_max(ex[0], ex[1], ex[2], ex[3]);
+    float esum = tree_sum(ey[0], ey[1], ey[2], ey[3]);
+    float fsum = tree_sum(ez[0], ez[1], ez[2], ez[3]);
+    if (sc.decision_mode == 1) { emax = 0.f; esum = 0.f; fsum = 0.f; }
+    float L;
+    if (esum > c.mmspt) { L = emax * s.dt; L = L / esum; } else L = 0.0f;
+    if (L > 6.0f) {
+      s.dt = s.dt * 0.5f;
+      if (s.dt < c.dt_min) s.dt = c.dt_min;
+      another = 0; reset = 1;
+    } else if (fsum > 0.0f) {
+      another = 1; reset = 0;
+    } else if (L < 0.5f) {
+      s.dt = s.dt * 2.0f;
+      if (s.dt > c.dt_max) s.dt = c.dt_max;
+      another = 0; reset = 1;
+    } else {
+      another = 1; reset = 0;
+    }
+  } else {
+    another = 1; reset = 0;
+  }
+  out_loop++;
+  if (out_loop >= sc.max_passes) { another = 0; reset = 1; s.c_fail = 1.f; }
+}
+
+// Commit of a step: counters, sample advance with output row (:14130-14203),
+// state shift and local time (:14212-14230).
+__device__ __forceinline__ void step_commit(const BmLaunch &a, const cm_bm_item &it, const ItemScal &c, ItemState &s,
+                                            int out_loop, int lane) {
+  s.c_steps += 1.0f;
+  s.c_dt += s.dt;
+  s.c_passes += (float)out_loop;
+  s.n_steps++;
+  s.n_passes += out_loop;
+  if (s.t > c.tso) {
+    s.t -= c.tso;
+    s.sx0 = s.sx1;
+    s.sx1 = in_at(a, (int64_t)s.in_s + 2);
+    if (s.in_s + 1 >= it.mosi) {
+      const float div_times = s.c_dt / s.c_steps;
+      const float div_j = s.c_passes / s.c_steps;
+      if (s.in_s >= it.write_start && s.in_s < a.rows) {
+        if (a.speeds)
+          *reinterpret_cast<float4 *>(a.speeds + (int64_t)s.in_s * CM_SECTIONS + 4 * lane) =
+              make_float4(s.nv[0], s.nv[1], s.nv[2], s.nv[3]);
+        if (lane == 0) {
+          if (a.failed_nodes) a.failed_nodes[s.in_s] = (int32_t)s.c_fail;
+          if (a.conv_time) a.conv_time[s.in_s] = div_times;
+          if (a.conv_jacobi) a.conv_jacobi[s.in_s] = div_j;
+          if (a.conv_loops) a.conv_loops[s.in_s] = s.c_steps;
+        }
+      }
+      if (s.c_fail > 0.f) s.fb += 1;
+      s.ctb += div_times / c.span;
+      s.cjb += div_j / c.span;
+      s.clb += s.c_steps / c.span;
+    }
+    s.c_fail = 0.f; s.c_dt = 0.f; s.c_steps = 0.f; s.c_passes = 0.f;
+    s.in_s++;
+  }
+  UNROLL for (int j = 0; j < 4; ++j) {
+    s.pd[j] = s.nd[j]; s.pv[j] = s.nv[j]; s.pa[j] = s.na[j]; s.ppsi[j] = s.npsi[j];
+    s.ppsid[j] = s.npsid[j]; s.pxis[j] = s.nxis[j]; s.pxi[j] = s.nxi[j];
+  }
+  s.ow_pd = s.ow_nd; s.ow_pv = s.ow_nv; s.ow_pa = s.ow_na;
+  s.t += s.dt;
+}
+
+__device__ __forceinline__ void item_final(const BmLaunch &a, const ItemState &s, int item_index, int lane) {
+  if (lane == 0) {
+    if (a.failed_blocks) a.failed_blocks[item_index] = s.fb;
+    if (a.conv_time_blocks) a.conv_time_blocks[item_index] = s.ctb;
+    if (a.conv_loops_blocks) a.conv_loops_blocks[item_index] = s.clb;
+    if (a.conv_jacobi_blocks) a.conv_jacobi_blocks[item_index] = s.cjb;
+    if (a.stats) {
+      a.stats[item_index].steps = s.n_steps;
+      a.stats[item_index].passes = s.n_passes;
+      a.stats[item_index].sweeps = s.n_sweeps;
+    }
+  }
+}
+
+// Extrapolated predictor-pass guess: p <- p + (dt/dt_prev)(p - p_prev) when
+// that lowers the residual (ties keep the previous pressure).  The previous
+// step's pressure and dt_prev live in LDS (pold[0..63], pold[64].x).  Returns
+// the squared residual of the plain guess; *rre gets the extrapolation's, and
+// p holds the extrapolation on return when keep_lin, else the choice.
+__device__ __forceinline__ float extrapolate_guess(const ItemConst &k, const float (&Y)[4], float dt, float4 *pold,
+                                                   int lane, float (&p)[4], float &rre, bool keep_lin) {
+  float *dt_prev = reinterpret_cast<float *>(pold + 64);
+  const float rho = dt / *dt_prev;
+  const float rr0 = resid2(Y, k.mf, k.low23, p);
+  const float4 po = pold[lane];
+  pold[lane] = make_float4(p[0], p[1], p[2], p[3]);
+  p[0] = p[0] + rho * (p[0] - po.x);
+  p[1] = p[1] + rho * (p[1] - po.y);
+  p[2] = p[2] + rho * (p[2] - po.z);
+  p[3] = p[3] + rho * (p[3] - po.w);
+  rre = resid2(Y, k.mf, k.low23, p);
+  if (!keep_lin && !(rre < rr0)) {
+    const float4 pk = pold[lane];
+    p[0] = pk.x; p[1] = pk.y; p[2] = pk.z; p[3] = pk.w;
+  }
+  if (lane == 0) *dt_prev = dt;
+  return rr0;
+}
+
+// One time block, include/cochlea.cuh:13620-14232.  TOL selects the
+// sweep-to-tolerance Jacobi; otherwise the reference's fixed 20/2 sweeps.
+// WS = 1 adds the extrapolated predictor-pass guess (tolerance mode only).
+template <bool TOL, int WS>
+__device__ void run_item(const BmLaunch &a, int item_index, int lane, float4 *pold) {
+  const cm_bm_item it = a.items[item_index];
+  ItemConst k;
+  ItemScal c;
+  ItemState s;
+  item_setup<TOL>(a, it, lane, k, c, s);
+  if (WS == 1) {
+    pold[lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (lane == 0) reinterpret_cast<float *>(pold + 64)[0] = s.dt;
+  }
+
+  while (s.in_s < it.mesi) {
     int out_loop = 0, reset = 1, another = 1;
     while (another) {
       float vref[4], aref[4];
-      UNROLL for (int j = 0; j < 4; ++j) { vref[j] = nv[j]; aref[j] = na[j]; }
-      if (out_loop == 0 || reset) {
-        out_loop = 0;
-        UNROLL for (int j = 0; j < 4; ++j) {
-          nxi[j] = pxis[j] * dt + pxi[j];
-          nv[j] = pa[j] * dt + pv[j];
-          nd[j] = ((pv[j] + nv[j]) * 0.5f) * dt + pd[j];
-          npsi[j] = ppsid[j] * dt + ppsi[j];
-        }
-        ow_nd = ow_pv * dt + ow_pd;
-        ow_nv = ow_pa * dt + ow_pv;
-      } else {
-        UNROLL for (int j = 0; j < 4; ++j) {
-          nv[j] = ((pa[j] + na[j]) * 0.5f) * dt + pv[j];
-          nd[j] = ((pv[j] + nv[j]) * 0.5f) * dt + pd[j];
-          nxi[j] = ((pxis[j] + nxis[j]) * 0.5f) * dt + pxi[j];
-          npsi[j] = ((ppsid[j] + npsid[j]) * 0.5f) * dt + ppsi[j];
-        }
-        ow_nd = ((ow_pv + ow_nv) * 0.5f) * dt + ow_pd;
-        ow_nv = ((ow_pa + ow_na) * 0.5f) * dt + ow_pv;
-      }
+      UNROLL for (int j = 0; j < 4; ++j) { vref[j] = s.nv[j]; aref[j] = s.na[j]; }
+      const bool predictor = out_loop == 0 || reset;
+      if (predictor) out_loop = 0;
+      pass_begin(s, predictor);
       float od[4], G[4], Y[4];
-      const bool aux_pass = out_loop < 2;
-      UNROLL for (int j = 0; j < 4; ++j) {
-        od[j] = nxi[j] - nd[j];
-        const float ohc_p = (gam[j] * Sohc[j]) * od[j];
-        if (aux_pass) {
-          float aux = cm_tanhf(al_over_as * npsi[j]);
-          const float dL = neg_as * aux;
-          aux = aux * aux - 1.0f;
-          aux = al * aux;
-          float n = (eta2 * nv[j] + eta1 * od[j]) - w_ohc * npsi[j];
-          n = Stm[j] * dL + (n * Rtm[j]) * aux;
-          const float ptmp = Stm[j] * nxi[j] + ohc_p;
-          n = ptmp - gam[j] * n;
-          float den = (gam[j] * aux) * eta2 - 1.0f;
-          den = Rtm[j] * den;
-          nxis[j] = n / den;
-          Ptm[j] = Rtm[j] * nxis[j] + Stm[j] * nxi[j];
-        }
-        float g = nv[j] * nv[j];
-        g = R[j] * (alpha_r * g + 1.0f);
-        g = g * nv[j] + S[j] * nd[j];
-        G[j] = -g - Ptm[j];
-        Y[j] = (dxdx * G[j]) * Q[j];
-      }
-      const float delta = t / Ts;
-      const float s = sx1 * delta + sx0 * (1.0f - delta);
-      const float Y0 = dxa0 * (((neg_a2 * ow_nv) - a1 * ow_nd) + Gme * s);
-      if (lane == 0) Y[0] = Y0;
-      if (lane == 63) Y[3] = 0.0f;
-
+      const float sf = pass_rhs(k, c, s, out_loop < 2, lane, od, G, Y);
       int J;
       if (TOL) {
-        // sweep until |Y - A p|_2 <= tol |Y|_2, checking every check_every sweeps
         const float yy = tree_sum(Y[0] * Y[0], Y[1] * Y[1], Y[2] * Y[2], Y[3] * Y[3]);
-        const float tau2 = sc.tol_rel * sc.tol_rel;
         if (WS == 1 && out_loop == 0) {
-          // predictor-pass guess: keep the linear extrapolation of the last
-          // two steps' pressures when its residual is smaller
-          // (in place: p is parked in LDS while the extrapolation is scored)
-          const float rho = dt / *dt_prev_lds;
-          const float rr0 = resid2(Y, mf, low23, p);
-          const float4 po = pold_lds[lane];
-          pold_lds[lane] = make_float4(p[0], p[1], p[2], p[3]);
-          p[0] = p[0] + rho * (p[0] - po.x);
-          p[1] = p[1] + rho * (p[1] - po.y);
-          p[2] = p[2] + rho * (p[2] - po.z);
-          p[3] = p[3] + rho * (p[3] - po.w);
-          const float rre = resid2(Y, mf, low23, p);
-          if (!(rre < rr0)) {
-            const float4 pk = pold_lds[lane];
-            p[0] = pk.x; p[1] = pk.y; p[2] = pk.z; p[3] = pk.w;
-          }
-          if (lane == 0) *dt_prev_lds = dt;
+          float rre;
+          (void)extrapolate_guess(k, Y, s.dt, pold, lane, s.p, rre, false);
         }
-        J = 0;
-        for (;;) {
-          const float rr = resid2(Y, mf, low23, p);
-          if (rr <= tau2 * yy || J >= sc.max_sweeps) break;
-          const int kk = sc.check_every < sc.max_sweeps - J ? sc.check_every : sc.max_sweeps - J;
-          jacobi_sweeps(Y, rsm, low23, p, kk);
-          J += kk;
-        }
+        J = jacobi_tol(a.sc, k, Y, yy, s.p);
       } else {
-        J = aux_pass ? sc.j_slow : sc.j_fast;
-        jacobi_sweeps(Y, rsm, low23, p, J);
+        J = out_loop < 2 ? a.sc.j_slow : a.sc.j_fast;
+        jacobi_sweeps(Y, k.rsm, k.low23, s.p, J);
       }
-      n_sweeps += J;
-      UNROLL for (int j = 0; j < 4; ++j) na[j] = (p[j] + G[j]) * rmass[j];
-      {
-        const float p0 = readlane(p[0], 0);
-        const float NN1 = neg_a2 * ow_nv;
-        const float NN2 = Gme * s + p0;
-        const float NN3 = neg_a1 * ow_nd;
-        ow_na = ((NN1 + NN2) + NN3) / sigma_ow;
-      }
-      UNROLL for (int j = 0; j < 4; ++j)
-        npsid[j] = (eta1 * od[j] + eta2 * (nxis[j] - nv[j])) + neg_w * npsi[j];
-
-      if (reset == 0) {
-        float ex[4], ey[4], ez[4];
-        UNROLL for (int j = 0; j < 4; ++j) {
-          ex[j] = __builtin_fabsf(aref[j] - na[j]);
-          ey[j] = __builtin_fabsf(nv[j] - vref[j]);
-          ez[j] = ey[j] - fmaxf(mtt, nv[j] * dt);
-        }
-        float emax = tree_max(ex[0], ex[1], ex[2], ex[3]);
-        float esum = tree_sum(ey[0], ey[1], ey[2], ey[3]);
-        float fsum = tree_sum(ez[0], ez[1], ez[2], ez[3]);
-        if (sc.decision_mode == 1) { emax = 0.f; esum = 0.f; fsum = 0.f; }
-        float L;
-        if (esum > mmspt) { L = emax * dt; L = L / esum; } else L = 0.0f;
-        if (L > 6.0f) {
-          dt = dt * 0.5f;
-          if (dt < dt_min) dt = dt_min;
-          another = 0; reset = 1;
-        } else if (fsum > 0.0f) {
-          another = 1; reset = 0;
-        } else if (L < 0.5f) {
-          dt = dt * 2.0f;
-          if (dt > dt_max) dt = dt_max;
-          another = 0; reset = 1;
-        } else {
-          another = 1; reset = 0;
-        }
-      } else {
-        another = 1; reset = 0;
-      }
-      out_loop++;
-      if (out_loop >= sc.max_passes) { another = 0; reset = 1; c_fail = 1.f; }
+      s.n_sweeps += J;
+      pass_finish(k, c, s, od, G, sf);
+      pass_control(a.sc, c, s, vref, aref, out_loop, reset, another);
     }
-    c_steps += 1.0f;
-    c_dt += dt;
-    c_passes += (float)out_loop;
-    n_steps++;
-    n_passes += out_loop;
-    if (t > tso) {
-      t -= tso;
-      sx0 = sx1;
-      sx1 = in_at(a, (int64_t)in_s + 2);
-      if (in_s + 1 >= it.mosi) {
-        const float div_times = c_dt / c_steps;
-        const float div_j = c_passes / c_steps;
-        if (in_s >= it.write_start && in_s < a.rows) {
-          if (a.speeds)
-            *reinterpret_cast<float4 *>(a.speeds + (int64_t)in_s * CM_SECTIONS + base) =
-                make_float4(nv[0], nv[1], nv[2], nv[3]);
-          if (lane == 0) {
-            if (a.failed_nodes) a.failed_nodes[in_s] = (int32_t)c_fail;
-            if (a.conv_time) a.conv_time[in_s] = div_times;
-            if (a.conv_jacobi) a.conv_jacobi[in_s] = div_j;
-            if (a.conv_loops) a.conv_loops[in_s] = c_steps;
-          }
-        }
-        if (c_fail > 0.f) fb += 1;
-        ctb += div_times / span;
-        cjb += div_j / span;
-        clb += c_steps / span;
-      }
-      c_fail = 0.f; c_dt = 0.f; c_steps = 0.f; c_passes = 0.f;
-      in_s++;
-    }
-    UNROLL for (int j = 0; j < 4; ++j) {
-      pd[j] = nd[j]; pv[j] = nv[j]; pa[j] = na[j]; ppsi[j] = npsi[j];
-      ppsid[j] = npsid[j]; pxis[j] = nxis[j]; pxi[j] = nxi[j];
-    }
-    ow_pd = ow_nd; ow_pv = ow_nv; ow_pa = ow_na;
-    t += dt;
+    step_commit(a, it, c, s, out_loop, lane);
   }
-  if (lane == 0) {
-    if (a.failed_blocks) a.failed_blocks[item_index] = fb;
-    if (a.conv_time_blocks) a.conv_time_blocks[item_index] = ctb;
-    if (a.conv_loops_blocks) a.conv_loops_blocks[item_index] = clb;
-    if (a.conv_jacobi_blocks) a.conv_jacobi_blocks[item_index] = cjb;
-    if (a.stats) {
-      a.stats[item_index].steps = n_steps;
-      a.stats[item_index].passes = n_passes;
-      a.stats[item_index].sweeps = n_sweeps;
-    }
-  }
+  item_final(a, s, item_index, lane);
 }
 
 template <int WAVES, bool TOL, int WS>
@@ -398,6 +495,243 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(3)))
   if (idx < a.n_items) run_item<TOL, WS>(a, idx, lane, pold_lds + (WS ? wave * 65 : 0));
 }
 
+// ------------------------------------------- learned warm start ---------
+// A workgroup of G waves integrates G time blocks in lockstep, one step per
+// round.  At each step's predictor pass every wave writes its features to
+// LDS and the G waves evaluate the 769 -> 512 -> 512 -> 256 GELU MLP together
+// on MFMA (v_mfma_f32_16x16x16_bf16, rows = blocks, padded to 16), with the
+// BF16 weights streamed from L2 in MFMA fragment order.  Each wave then takes
+// the plain, extrapolated or learned guess with the smallest residual and
+// iterates Jacobi to tolerance as in the fused kernel.
+constexpr int WS_XS = 792;    // LDS row stride (bf16) of the feature tile (K = 784 used)
+constexpr int WS_HS = 520;    // row stride (bf16) of hidden tiles
+constexpr int WS_OS = 260;    // row stride (float) of the output tile
+
+typedef __attribute__((__vector_size__(4 * sizeof(short)))) short ws_bf16x4;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float ws_f32x4;
+
+// One dense layer: out[16][N] = act(A[16][K] W + b) with NT = N/16 column tiles
+// shared out among the G waves (up to three tiles per wave, sharing the A
+// fragment of each k step).
+// Up to three column tiles (t1, t2 < 0: absent) of one layer, sharing the A
+// fragment of each k step.
+template <int KT, int NT, bool GELU>
+__device__ __forceinline__ void ws_tiles(const uint16_t *A, int sa, const uint2 *__restrict__ W,
+                                         const float *__restrict__ bias, void *out, int so, int lane, int t0, int t1,
+                                         int t2) {
+  const bool v1 = t1 >= 0, v2 = t2 >= 0;
+  ws_f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = acc0, acc2 = acc0;
+  const uint16_t *arow = A + (lane & 15) * sa + 4 * (lane >> 4);
+#pragma unroll 4
+  for (int kt = 0; kt < KT; ++kt) {
+    const uint2 av = *reinterpret_cast<const uint2 *>(arow + kt * 16);
+    const ws_bf16x4 af = __builtin_bit_cast(ws_bf16x4, av);
+    const uint2 *wk = W + (size_t)kt * NT * 64 + lane;
+    const uint2 b0 = wk[(size_t)t0 * 64];
+    acc0 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, __builtin_bit_cast(ws_bf16x4, b0), acc0, 0, 0, 0);
+    if (v1) {
+      const uint2 b1 = wk[(size_t)t1 * 64];
+      acc1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, __builtin_bit_cast(ws_bf16x4, b1), acc1, 0, 0, 0);
+    }
+    if (v2) {
+      const uint2 b2 = wk[(size_t)t2 * 64];
+      acc2 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, __builtin_bit_cast(ws_bf16x4, b2), acc2, 0, 0, 0);
+    }
+  }
+  // lane holds rows 4*(lane/16)+q of column 16*t + lane%16
+  const int r0 = 4 * (lane >> 4), cl = lane & 15;
+  auto store = [&](const ws_f32x4 &acc, int t) {
+    const int col = 16 * t + cl;
+    const float bv = bias[col];
+    UNROLL for (int q = 0; q < 4; ++q) {
+      const float v = acc[q] + bv;
+      if (GELU)
+        reinterpret_cast<uint16_t *>(out)[(r0 + q) * so + col] = cm_f32_to_bf16(cm_gelu(v));
+      else
+        reinterpret_cast<float *>(out)[(r0 + q) * so + col] = v;
+    }
+  };
+  store(acc0, t0);
+  if (v1) store(acc1, t1);
+  if (v2) store(acc2, t2);
+}
+
+template <int G, int KT, int NT, bool GELU>
+__device__ __noinline__ void ws_layer(const uint16_t *A, int sa, const uint2 *__restrict__ W,
+                                         const float *__restrict__ bias, void *out, int so, int wave, int lane) {
+  // wave w computes tiles w, w + G, w + 2G, then w + 3G, ... three at a time
+  for (int t0 = wave; t0 < NT; t0 += 3 * G) {
+    const int t1 = t0 + G, t2 = t0 + 2 * G;
+    ws_tiles<KT, NT, GELU>(A, sa, W, bias, out, so, lane, t0, t1 < NT ? t1 : -1, t2 < NT ? t2 : -1);
+  }
+}
+
+__device__ __forceinline__ float rms256(float x0, float x1, float x2, float x3) {
+  return __builtin_sqrtf(tree_sum(x0 * x0, x1 * x1, x2 * x2, x3 * x3) * (1.0f / 256.0f)) + 1e-30f;
+}
+
+template <int G>
+__global__ __launch_bounds__(64 * G) void bm_ws_kernel(BmLaunch a, WsArgs w) {
+  __shared__ __align__(16) uint16_t Xs[16 * WS_XS];
+  __shared__ __align__(16) uint16_t H1s[16 * WS_HS];
+  __shared__ __align__(16) uint16_t H2s[16 * WS_HS];      // also the output tile (16 x WS_OS floats)
+  __shared__ float4 pold_lds[G * 65];
+  __shared__ int active_lds[2][G];                         // double-buffered by round parity
+
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int idx = blockIdx.x * G + wave;
+  const bool valid = idx < a.n_items;
+  float4 *pold = pold_lds + wave * 65;
+
+  // zero the feature tile once (rows >= G stay zero)
+  for (int e = threadIdx.x; e < 16 * WS_XS; e += 64 * G) Xs[e] = 0;
+
+  cm_bm_item it;
+  ItemConst k;
+  ItemScal c;
+  ItemState s;
+  if (valid) {
+    it = a.items[idx];
+  } else {
+    it.start = 0; it.mosi = 0; it.mesi = 0; it.write_start = 0; it.tol = 0.f; it.m1tol = 0.f; it.set = 0;
+    it.pad = 0;
+  }
+  item_setup<true>(a, it, lane, k, c, s);
+  pold[lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (lane == 0) reinterpret_cast<float *>(pold + 64)[0] = s.dt;
+  __syncthreads();
+
+  const float inv_256 = 1.0f / 256.0f;
+  for (int round = 0;; ++round) {
+    const bool active = valid && s.in_s < it.mesi;
+    float rr0 = 0.f, rre = 0.f, rr_scale = 0.f;
+    if (active) {
+      float od[4], G4[4], Y[4];
+      pass_begin(s, true);
+      const float sf = pass_rhs(k, c, s, true, lane, od, G4, Y);
+      rr0 = extrapolate_guess(k, Y, s.dt, pold, lane, s.p, rre, true);   // s.p = p_lin
+      float r[4];
+      resid_vec(Y, k.mf, k.low23, s.p, r);
+      rr_scale = __builtin_sqrtf(rre * inv_256) + 1e-30f;
+      const float4 pk = pold[lane];   // plain guess (previous pressure)
+      const float inv_d = 1.0f / rms256(s.nd[0], s.nd[1], s.nd[2], s.nd[3]);
+      const float inv_v = 1.0f / rms256(s.nv[0], s.nv[1], s.nv[2], s.nv[3]);
+      const float inv_r = 1.0f / rr_scale;
+      const float inv_p = 1.0f / rms256(pk.x, pk.y, pk.z, pk.w);
+      float f[13];
+      UNROLL for (int j = 0; j < 4; ++j) {
+        f[j] = s.nd[j] * inv_d;
+        f[4 + j] = s.nv[j] * inv_v;
+        f[8 + j] = r[j] * inv_r;
+      }
+      f[12] = sf * inv_p;
+      uint16_t *xr = Xs + wave * WS_XS;
+      UNROLL for (int j = 0; j < 4; ++j) {
+        xr[4 * lane + j] = cm_f32_to_bf16(f[j]);
+        xr[256 + 4 * lane + j] = cm_f32_to_bf16(f[4 + j]);
+        xr[512 + 4 * lane + j] = cm_f32_to_bf16(f[8 + j]);
+      }
+      if (lane == 0) xr[768] = cm_f32_to_bf16(f[12]);
+      if (w.capture && (s.n_steps % w.stride) == 0 && s.n_steps / w.stride < w.per_item) {   // slot of this step
+        float *rec = w.records + ((int64_t)idx * w.per_item + s.n_steps / w.stride) * CM_WS_RECORD;
+        *reinterpret_cast<float4 *>(rec + 4 * lane) = make_float4(f[0], f[1], f[2], f[3]);
+        *reinterpret_cast<float4 *>(rec + 256 + 4 * lane) = make_float4(f[4], f[5], f[6], f[7]);
+        *reinterpret_cast<float4 *>(rec + 512 + 4 * lane) = make_float4(f[8], f[9], f[10], f[11]);
+        *reinterpret_cast<float4 *>(rec + CM_WS_REC_Y + 4 * lane) = make_float4(Y[0], Y[1], Y[2], Y[3]);
+        *reinterpret_cast<float4 *>(rec + CM_WS_REC_PLIN + 4 * lane) = make_float4(s.p[0], s.p[1], s.p[2], s.p[3]);
+        if (lane == 0) {
+          rec[768] = f[12];
+          rec[769] = rr_scale;
+          rec[770] = (float)s.n_steps;
+          rec[771] = 0.f;
+        }
+      }
+    }
+    // a fast wave may write the next round's flag before a slow one has read
+    // this round's, so the flags alternate between two slots
+    if (lane == 0) active_lds[round & 1][wave] = active ? 1 : 0;
+    __syncthreads();
+    int any = 0;
+    for (int g = 0; g < G; ++g) any |= active_lds[round & 1][g];
+    if (!any) break;
+    if (w.use_net) {
+      // X -> H2 -> H1 -> output tile in H2's space (H2 is dead once layer 2 ends)
+      ws_layer<G, 49, 32, true>(Xs, WS_XS, w.w1, w.b1, H2s, WS_HS, wave, lane);
+      __syncthreads();
+      ws_layer<G, 32, 32, true>(H2s, WS_HS, w.w2, w.b2, H1s, WS_HS, wave, lane);
+      __syncthreads();
+      ws_layer<G, 32, 16, false>(H1s, WS_HS, w.w3, w.b3, H2s, WS_OS, wave, lane);
+      __syncthreads();
+    }
+    // The section constants are re-read from L1/L2 here, so that they are not
+    // held in registers across the network evaluation (the memory clobber keeps
+    // the compiler from reusing the earlier loads).
+    asm volatile("" ::: "memory");
+    load_consts<true>(a, it, lane, k);
+    if (active) {
+      // the predictor pass's right-hand side again (same inputs, same values):
+      // cheaper than holding it in registers across the network evaluation
+      float od[4], G4[4], Y[4];
+      const float sf = pass_rhs(k, c, s, true, lane, od, G4, Y);
+      const float yy = tree_sum(Y[0] * Y[0], Y[1] * Y[1], Y[2] * Y[2], Y[3] * Y[3]);
+      if (w.use_net) {
+        // learned guess: p_lin + rms(r_lin) c_out S(y), S = [1,2,1]/4 (edges replicated)
+        const float4 y4 = *reinterpret_cast<const float4 *>(reinterpret_cast<const float *>(H2s) + wave * WS_OS +
+                                                             4 * lane);
+        if (w.capture && (s.n_steps % w.stride) == 0 && s.n_steps / w.stride < w.per_item)
+          *reinterpret_cast<float4 *>(w.records + ((int64_t)idx * w.per_item + s.n_steps / w.stride) * CM_WS_RECORD +
+                                      CM_WS_REC_OUT + 4 * lane) = y4;
+        float left = wave_shr1(y4.w), right = wave_shl1(y4.x);
+        if (lane == 0) left = y4.x;
+        if (lane == 63) right = y4.w;
+        const float sc_out = rr_scale * w.c_out;
+        float ph[4];
+        ph[0] = s.p[0] + sc_out * ((0.25f * left + 0.5f * y4.x) + 0.25f * y4.y);
+        ph[1] = s.p[1] + sc_out * ((0.25f * y4.x + 0.5f * y4.y) + 0.25f * y4.z);
+        ph[2] = s.p[2] + sc_out * ((0.25f * y4.y + 0.5f * y4.z) + 0.25f * y4.w);
+        ph[3] = s.p[3] + sc_out * ((0.25f * y4.z + 0.5f * y4.w) + 0.25f * right);
+        const float rrh = resid2(Y, k.mf, k.low23, ph);
+        if (rrh < rre && rrh < rr0) {
+          UNROLL for (int j = 0; j < 4; ++j) s.p[j] = ph[j];
+        } else if (!(rre < rr0)) {
+          const float4 pk = pold[lane];
+          s.p[0] = pk.x; s.p[1] = pk.y; s.p[2] = pk.z; s.p[3] = pk.w;
+        }
+      } else if (!(rre < rr0)) {
+        const float4 pk = pold[lane];
+        s.p[0] = pk.x; s.p[1] = pk.y; s.p[2] = pk.z; s.p[3] = pk.w;
+      }
+      // rest of the predictor pass, then the corrector passes of this step
+      int out_loop = 0, reset = 1, another = 1;
+      {
+        const int J = jacobi_tol(a.sc, k, Y, yy, s.p);
+        s.n_sweeps += J;
+        pass_finish(k, c, s, od, G4, sf);
+        float vref[4], aref[4];   // unused: step control is skipped on the predictor pass
+        UNROLL for (int j = 0; j < 4; ++j) { vref[j] = 0.f; aref[j] = 0.f; }
+        pass_control(a.sc, c, s, vref, aref, out_loop, reset, another);
+      }
+      // vref/aref of the first corrector pass are the predictor pass's values
+      while (another) {
+        float vref[4], aref[4];
+        UNROLL for (int j = 0; j < 4; ++j) { vref[j] = s.nv[j]; aref[j] = s.na[j]; }
+        const bool predictor = out_loop == 0 || reset;
+        if (predictor) out_loop = 0;
+        pass_begin(s, predictor);
+        float od2[4], G2[4], Y2[4];
+        const float sf2 = pass_rhs(k, c, s, out_loop < 2, lane, od2, G2, Y2);
+        const float yy2 = tree_sum(Y2[0] * Y2[0], Y2[1] * Y2[1], Y2[2] * Y2[2], Y2[3] * Y2[3]);
+        const int J = jacobi_tol(a.sc, k, Y2, yy2, s.p);
+        s.n_sweeps += J;
+        pass_finish(k, c, s, od2, G2, sf2);
+        pass_control(a.sc, c, s, vref, aref, out_loop, reset, another);
+      }
+      step_commit(a, it, c, s, out_loop, lane);
+    }
+  }
+  if (valid) item_final(a, s, idx, lane);
+}
 
 }  // namespace cm
 
@@ -426,4 +760,21 @@ extern "C" int cm_launch_bm(const cm::BmLaunch *a, int waves_per_block, int grid
   }
 #undef CM_BM_CASE
   return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+// Lockstep groups of G blocks: the smallest G in {8, 10, 12} whose groups fit
+// one per CU, else 12.  Returns G (> 0) or a negative error.
+extern "C" int cm_launch_bm_ws(const cm::BmLaunch *a, const cm::WsArgs *w, int n_cus, hipStream_t stream) {
+  if (a->sc.jacobi_mode != 1) return -1;
+  const int n = a->n_items;
+  int G = 12;
+  if ((n + 7) / 8 <= n_cus) G = 8;
+  else if ((n + 9) / 10 <= n_cus) G = 10;
+  const dim3 grid((n + G - 1) / G);
+  switch (G) {
+    case 8: hipLaunchKernelGGL((cm::bm_ws_kernel<8>), grid, dim3(64 * 8), 0, stream, *a, *w); break;
+    case 10: hipLaunchKernelGGL((cm::bm_ws_kernel<10>), grid, dim3(64 * 10), 0, stream, *a, *w); break;
+    default: hipLaunchKernelGGL((cm::bm_ws_kernel<12>), grid, dim3(64 * 12), 0, stream, *a, *w); break;
+  }
+  return hipGetLastError() == hipSuccess ? G : -2;
 }

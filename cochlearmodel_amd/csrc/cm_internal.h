@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <stdint.h>
+
 #include "cm_types.h"
 #include "../../include/cochlea_amd.h"
 
@@ -28,6 +30,43 @@ struct BmLaunch {
   cm_item_stats *stats;
 };
 
+// Learned warm start of the predictor-pass pressure solve (warm_start = 2) and
+// the capture of its training records (capture != 0, warm_start = 1 guesses).
+struct WsArgs {
+  const uint2 *__restrict__ w1;      // MFMA-packed bf16 [49][32][64] x 4 (K 769 padded to 784)
+  const uint2 *__restrict__ w2;      // [32][32][64] x 4
+  const uint2 *__restrict__ w3;      // [32][16][64] x 4
+  const float *__restrict__ b1, *__restrict__ b2, *__restrict__ b3;
+  float c_out;                       // target scale: p = p_lin + rms(r_lin) * c_out * S(y)
+  int32_t use_net;                   // 1: evaluate the network and offer its guess
+  int32_t capture;                   // 1: write records (training data; network output too when use_net)
+  float *records;                    // capture: [items * per_item][CM_WS_RECORD] floats
+  int32_t stride;                    // capture every stride-th committed step
+  int32_t per_item;                  // capture slots per item
+};
+
+// record layout (floats): features [0, 769), rms(r_lin) at 769, committed step
+// index at 770, Y at [772, 1028), p_lin at [1028, 1284), network output at
+// [1288, 1544) when the network ran
+constexpr int CM_WS_IN = 769;
+constexpr int CM_WS_RECORD = 1544;
+constexpr int CM_WS_REC_Y = 772;
+constexpr int CM_WS_REC_PLIN = 1028;
+constexpr int CM_WS_REC_OUT = 1288;
+
+__device__ __forceinline__ uint16_t cm_f32_to_bf16(float x) {
+  // round to nearest even; NaN kept quiet
+  uint32_t u = __float_as_uint(x);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+// GELU, tanh form, shared by training and in-solver inference
+__device__ __forceinline__ float cm_gelu(float x) {
+  return 0.5f * x * (1.0f + tanhf(0.7978845608f * (x + 0.044715f * x * x * x)));
+}
+
 struct JndArgs {
   const float *lambda;      // 3 x rows x 256
   int64_t rows;             // plane stride
@@ -51,6 +90,10 @@ struct JndArgs {
 };
 
 void set_error(const char *fmt, ...);
+int device_cus();
+// learned warm start state (ws_abi.hip)
+bool ws_wants_group_kernel(const cm_bm_scalars &sc);
+int ws_args(const cm_bm_scalars &sc, int n_items, WsArgs *out);
 int check(hipError_t e, const char *what);
 hipStream_t stream();
 float *bm_speeds_ptr();
@@ -62,6 +105,8 @@ int bm_ensure_input(size_t n);
 }  // namespace cm
 
 extern "C" int cm_launch_bm(const cm::BmLaunch *a, int waves_per_block, int grid_blocks, hipStream_t stream);
+// lockstep-group launch of the warm-start kernel; returns the group size used
+extern "C" int cm_launch_bm_ws(const cm::BmLaunch *a, const cm::WsArgs *w, int n_cus, hipStream_t stream);
 extern "C" int cm_launch_ihc_lambda(const float *d_bm, int64_t rows, const cm_ihc_interval *d_ivs,
                                     const cm_ihc_interval *h_ivs, int n_ivs, const cm_ihc_interval *d_iir_ivs,
                                     int n_iir_ivs, const double *d_ac_b, const double *d_ac_bpad,

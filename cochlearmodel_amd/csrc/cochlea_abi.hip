@@ -91,6 +91,17 @@ static int ensure_events() {
   return 0;
 }
 
+int device_cus() {
+  static int cus = 0;
+  if (!cus) {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (cus <= 0) cus = 1;
+  }
+  return cus;
+}
+
 float *bm_speeds_ptr() { return g_bm.speeds.ptr; }
 size_t bm_result_nodes() { return g_bm.speeds.cap; }
 float *bm_input_ptr() { return g_bm.input.ptr; }
@@ -166,7 +177,15 @@ int launch_bm_items(const cm_bm_scalars &sc, const float *d_input, int64_t input
   if (n_items <= 0) return 0;
   if (ensure_events()) return -1;
   if (kernel_ms && check(hipEventRecord(g_bm.ev0, stream()), "hipEventRecord")) return -1;
-  const int rc = cm_launch_bm(&a, waves_per_block, grid, stream());
+  int rc;
+  if (ws_wants_group_kernel(sc)) {
+    WsArgs w;
+    if (ws_args(sc, n_items, &w)) return -3;
+    rc = cm_launch_bm_ws(&a, &w, device_cus(), stream());
+    rc = rc > 0 ? 0 : rc;
+  } else {
+    rc = cm_launch_bm(&a, waves_per_block, grid, stream());
+  }
   if (rc) {
     set_error("BM kernel launch failed (%d): %s", rc, hipGetErrorString(hipGetLastError()));
     return -1;
@@ -414,6 +433,10 @@ int cm_bm_run(const cm_bm_scalars *sc, const float *d_input, int64_t input_len, 
   }
   if (sc->j_fast < 0 || sc->j_slow < 0 || sc->max_passes < 1) {
     set_error("cm_bm_run: invalid Jacobi/pass counts");
+    return -3;
+  }
+  if (sc->jacobi_mode == 1 && (sc->warm_start < 0 || sc->warm_start > 2 || sc->check_every < 1)) {
+    set_error("cm_bm_run: warm_start must be 0, 1 or 2 and check_every >= 1");
     return -3;
   }
   float ms = 0.f;

@@ -131,7 +131,7 @@ class WarmStartMLP:
             check(L.cm_adam(ptr(self.b[i]), None, ptr(self.gb[i]), ptr(self.mb[i]), ptr(self.vb[i]),
                             self.b[i].numel(), lr, b1, b2, eps, self.t, 0.0), "adam")
 
-    def train(self, X, T, epochs=3, batch=1024, lr=1e-3, seed=0, log=None):
+    def train(self, X, T, epochs=3, batch=1024, lr=1e-3, seed=0, log=None, decay=0.5):
         """Mini-batch Adam over device tensors X (N, 769), T (N, 256)."""
         torch = self.torch
         n = X.shape[0]
@@ -140,7 +140,7 @@ class WarmStartMLP:
         for ep in range(epochs):
             perm = torch.randperm(n, generator=g).to(self.dev)
             tot, cnt = 0.0, 0
-            lr_ep = lr * (0.5 ** ep)
+            lr_ep = lr * (decay ** ep)
             for s0 in range(0, n - batch + 1, batch):
                 idx = perm[s0:s0 + batch]
                 xb = X.index_select(0, idx).contiguous()

@@ -1,0 +1,183 @@
+"""Learned warm start of the predictor-pass Jacobi solve: capture of training
+records from the solver itself, exact targets, training on the GPU with the
+library's BF16-MFMA kernels, upload into the solver and evaluation against the
+plain and extrapolated guesses.  No reference counterpart.
+
+    python -m cochlearmodel_amd.warmstart train --out cochlearmodel_amd/data/warmstart_mlp.npz
+    python -m cochlearmodel_amd.warmstart eval --model cochlearmodel_amd/data/warmstart_mlp.npz
+
+Network input per predictor pass (769 values, each block RMS-normalised):
+predictor displacement d, predictor velocity v, the residual r_lin of the
+extrapolated pressure p_lin = p + (dt/dt_prev)(p - p_prev), and the input
+sample s (scaled by the previous pressure's RMS).  Output y (256): the
+correction, applied as p_lin + rms(r_lin) c_out [1,2,1]/4 * y.  Target:
+(A^-1 Y - p_lin) / (rms(r_lin) c_out) with A^-1 Y the exact FP64 solve.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import _lib, solver
+from ._lib import check, lib, ptr
+from .batch import ToneBatch, random_tones
+from .mlp import SIZES, WarmStartMLP
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+MODEL_PATH = os.path.join(HERE, "data", "warmstart_mlp.npz")
+FS = 44100.0
+TRAIN_SEED = 20261202 + 51     # training tones
+EVAL_SEED = 20261202 + 52      # held-out tones
+
+
+def tol_scalars(tau: float = 1e-6, warm_start: int = 1):
+    sc = solver.default_scalars(FS)
+    sc.jacobi_mode, sc.tol_rel, sc.check_every, sc.max_sweeps, sc.warm_start = 1, tau, 4, 1000, warm_start
+    return sc
+
+
+def capture(tb: ToneBatch, tau: float, stride: int, per_item: int):
+    """Run the batch once with the extrapolated guess, recording every
+    stride-th step's predictor pass; returns the filled records (device)."""
+    import torch
+    rec = torch.full((tb.TB * per_item, _lib.WS_RECORD), -1.0, dtype=torch.float32, device="cuda")
+    check(lib().cm_ws_set_capture(ptr(rec), rec.shape[0], stride, per_item), "cm_ws_set_capture")
+    try:
+        tb.run_bm(tol_scalars(tau, 1), timed=False)
+    finally:
+        lib().cm_ws_set_capture(None, 0, 0, 0)
+    torch.cuda.synchronize()
+    keep = rec[:, 770] > 0          # slot written, and not the zero-state first step
+    return rec[keep].contiguous()
+
+
+def targets(sec, rec, c_out: float = 1.0):
+    import torch
+    T = torch.empty((rec.shape[0], 256), dtype=torch.float32, device="cuda")
+    check(lib().cm_ws_targets(ptr(sec), ptr(rec), rec.shape[0], C.c_float(c_out), ptr(T)), "cm_ws_targets")
+    return T
+
+
+def upload(params, c_out: float):
+    """Model into the solver (cm_ws_load); params = [W1, b1, W2, b2, W3, b3]."""
+    p = [np.ascontiguousarray(x, np.float32) for x in params]
+    for i, (kin, kout) in enumerate(zip(SIZES[:-1], SIZES[1:])):
+        assert p[2 * i].shape == (kin, kout) and p[2 * i + 1].shape == (kout,)
+    check(lib().cm_ws_load(ptr(p[0]), ptr(p[1]), ptr(p[2]), ptr(p[3]), ptr(p[4]), ptr(p[5]), C.c_float(c_out)),
+          "cm_ws_load")
+
+
+def save_model(path, params, c_out, meta):
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+    np.savez_compressed(path, W1=params[0], b1=params[1], W2=params[2], b2=params[3], W3=params[4], b3=params[5],
+                        c_out=np.float32(c_out), meta=json.dumps(meta))
+
+
+def load_model(path=MODEL_PATH):
+    z = np.load(path, allow_pickle=False)
+    params = [z[k].astype(np.float32) for k in ("W1", "b1", "W2", "b2", "W3", "b3")]
+    return params, float(z["c_out"]), json.loads(str(z["meta"]))
+
+
+def build_training_set(n_signals: int, batch_signals: int, stride: int, per_item: int, tau: float, seed: int,
+                       log=print):
+    import torch
+    Xs, Ts = [], []
+    for b in range(n_signals // batch_signals):
+        f, db = random_tones(batch_signals, seed + b)
+        tb = ToneBatch(f, db, 0.2, with_lambda=False)
+        tb.generate_input()
+        rec = capture(tb, tau, stride, per_item)
+        Ts.append(targets(tb.bm.sec, rec))
+        Xs.append(rec[:, :SIZES[0]].clone())
+        log(f"batch {b}: {rec.shape[0]} records")
+        del tb, rec
+        torch.cuda.empty_cache()
+    return torch.cat(Xs), torch.cat(Ts)
+
+
+def train(n_signals=2048, batch_signals=256, stride=1024, per_item=32, tau=1e-6, epochs=24, lr=1e-3, decay=0.85,
+          seed=TRAIN_SEED, log=print):
+    import torch
+    t0 = time.time()
+    X, T = build_training_set(n_signals, batch_signals, stride, per_item, tau, seed, log)
+    finite = torch.isfinite(X).all(1) & torch.isfinite(T).all(1)
+    X, T = X[finite].contiguous(), T[finite].contiguous()
+    c_out = float(torch.sqrt((T.double() ** 2).mean()).item())
+    T = (T / c_out).contiguous()
+    log(f"{X.shape[0]} records in {time.time() - t0:.1f} s, c_out {c_out:.4e}")
+    net = WarmStartMLP(seed=1, device="cuda")
+    t1 = time.time()
+    hist = net.train(X, T, epochs=epochs, batch=1024, lr=lr, decay=decay, log=log)
+    torch.cuda.synchronize()
+    meta = {"n_signals": n_signals, "records": int(X.shape[0]), "stride": stride, "tau": tau, "epochs": epochs,
+            "lr": lr, "decay": decay, "seed": seed, "final_mse": hist[-1], "train_s": time.time() - t1,
+            "features": "d, v, r_lin (RMS-normalised), s / rms(p_prev)",
+            "target": "(A^-1 Y - p_lin) / (rms(r_lin) c_out)"}
+    return net.params_numpy(), c_out, meta, hist
+
+
+def evaluate(params, c_out, n_signals=256, batch_signals=256, tau=1e-6, seed=EVAL_SEED, log=print):
+    """Held-out tones: BM time and sweeps/step of the plain, extrapolated and
+    learned guesses on the same resident input."""
+    import torch
+    upload(params, c_out)
+    res = {k: {"bm_ms": 0.0, "steps": 0, "passes": 0, "sweeps": 0} for k in ("plain", "extrapolated", "learned")}
+    for b in range(max(1, n_signals // batch_signals)):
+        f, db = random_tones(batch_signals, seed + b)
+        tb = ToneBatch(f, db, 0.2, with_lambda=False)
+        tb.generate_input()
+        for name, ws in (("plain", 0), ("extrapolated", 1), ("learned", 2)):
+            ms = tb.run_bm(tol_scalars(tau, ws), timed=True)
+            torch.cuda.synchronize()
+            st, pa, sw = tb.stats()
+            r = res[name]
+            r["bm_ms"] += ms
+            r["steps"] += st
+            r["passes"] += pa
+            r["sweeps"] += sw
+            log(f"batch {b} {name}: {ms:.1f} ms, {sw / st:.2f} sweeps/step")
+        del tb
+        torch.cuda.empty_cache()
+    out = {}
+    for k, r in res.items():
+        out[k] = {"bm_ms": r["bm_ms"], "section_steps_per_s": 256 * r["steps"] / (r["bm_ms"] * 1e-3),
+                  "jacobi_iters_per_step": r["sweeps"] / r["steps"], "passes_per_step": r["passes"] / r["steps"],
+                  "committed_steps": r["steps"]}
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("cmd", choices=("train", "eval"))
+    ap.add_argument("--out", default=MODEL_PATH)
+    ap.add_argument("--model", default=MODEL_PATH)
+    ap.add_argument("--signals", type=int, default=2048)
+    ap.add_argument("--eval-signals", type=int, default=256)
+    ap.add_argument("--epochs", type=int, default=24)
+    ap.add_argument("--stride", type=int, default=1024)
+    ap.add_argument("--tau", type=float, default=1e-6)
+    args = ap.parse_args(argv)
+    import torch
+    torch.cuda.set_device(0)
+    log = lambda m: print(m, flush=True)
+    if args.cmd == "train":
+        params, c_out, meta, hist = train(n_signals=args.signals, stride=args.stride, tau=args.tau,
+                                          epochs=args.epochs, log=log)
+        save_model(args.out, params, c_out, meta)
+        log(json.dumps({"saved": args.out, **meta}))
+        ev = evaluate(params, c_out, n_signals=args.eval_signals, tau=args.tau, log=log)
+    else:
+        params, c_out, meta = load_model(args.model)
+        ev = evaluate(params, c_out, n_signals=args.eval_signals, tau=args.tau, log=log)
+    print(json.dumps({"evaluation": ev, "held_out_signals": args.eval_signals}), flush=True)
+
+
+if __name__ == "__main__":
+    sys.exit(main())

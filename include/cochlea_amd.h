@@ -235,6 +235,25 @@ int cm_to_bf16(const float *x, void *y, int64_t n);
 int cm_colsum(const float *dZ, float *db, int M, int N);
 /* dY = 2 (Y - T) / (M N) (fp32 and bf16); row_loss[m] = sum_n (Y - T)^2 */
 int cm_mse_grad(const float *Y, const float *T, float *dY, void *dYbf, float *row_loss, int M, int N);
+/* In-solver use (cm_bm_scalars.warm_start = 2, jacobi_mode = 1): lockstep
+ * groups of time blocks evaluate the network at every predictor pass; the
+ * guess p_lin + rms(r_lin) c_out S(y) (S = [1,2,1]/4) competes with the
+ * previous pressure and its extrapolation p_lin by residual.
+ * Upload a model: W1 769x512, W2 512x512, W3 512x256 row-major fp32 (packed to
+ * bf16 MFMA fragments on the host), biases fp32, target scale c_out. */
+int cm_ws_load(const float *W1, const float *b1, const float *W2, const float *b2, const float *W3, const float *b3,
+               float c_out);
+/* Training records: with warm_start 1 or 2 (jacobi_mode 1) every cm_bm_run
+ * also writes, for every stride-th committed step of item i (slot
+ * i * per_item + step / stride), the network input (769 floats at 0),
+ * rms(r_lin) (769), the step index (770), Y (256 at 772), p_lin (256 at 1028)
+ * and, with warm_start 2, the network output (256 at 1288); 1544 floats per
+ * record.  d_records = NULL switches capture off. */
+int cm_ws_set_capture(float *d_records, int64_t n_records, int stride, int per_item);
+/* Exact training targets: (A^-1 Y - p_lin) / (rms(r_lin) c_out) per record, A
+ * from the m row of d_sec, FP64 inverse.  d_targets: n x 256 floats. */
+int cm_ws_targets(const float *d_sec, const float *d_records, int64_t n, float c_out, float *d_targets);
+int cm_ws_release(void);
 /* Adam step on fp32 master weights w (+ decoupled-free L2 term wd * w in the
  * gradient); refreshes the bf16 copy wbf when it is not NULL */
 int cm_adam(float *w, void *wbf, const float *g, float *m1, float *m2, int64_t n, float lr, float b1, float b2,

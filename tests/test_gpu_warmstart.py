@@ -1,0 +1,180 @@
+"""Learned warm start on the GPU: the lockstep-group kernel against the fused
+kernel (bit-exact where they must agree), the captured training records, the
+exact targets, the in-solver MFMA inference against the numpy oracle, and the
+trained network's gain over the plain and extrapolated guesses."""
+import numpy as np
+import pytest
+
+from oracle import mlp_oracle as mo
+
+FS = 44100.0
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _cuda():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def _batch(n=6, dur=0.06, seed=5):
+    from cochlearmodel_amd.batch import ToneBatch, random_tones
+    f, db = random_tones(n, seed)
+    tb = ToneBatch(f, db, dur, with_lambda=False)
+    tb.generate_input()
+    return tb
+
+
+def _run(tb, sc):
+    tb.run_bm(sc, timed=False)
+    import torch
+    torch.cuda.synchronize()
+    return tb.d_speeds.cpu().numpy().copy(), tb.d_stats.view(tb.TB, 3).cpu().numpy().copy()
+
+
+def _matrix(sec):
+    m = sec[8].astype(np.float64)
+    A = np.diag(m)
+    A += np.diag(np.r_[np.ones(253), 0.0, 0.0], -1)
+    A += np.diag(np.ones(255), 1)
+    return A
+
+
+def _capture_run(tb, sc, stride, per_item):
+    import torch
+    from cochlearmodel_amd import _lib
+    rec = torch.full((tb.TB * per_item, _lib.WS_RECORD), -1.0, dtype=torch.float32, device="cuda")
+    _lib.check(_lib.lib().cm_ws_set_capture(_lib.ptr(rec), rec.shape[0], stride, per_item))
+    try:
+        v, st = _run(tb, sc)
+    finally:
+        _lib.lib().cm_ws_set_capture(None, 0, 0, 0)
+    return v, st, rec.cpu().numpy()
+
+
+def test_capture_mode_matches_fused_extrapolated_kernel():
+    _cuda()
+    from cochlearmodel_amd import solver
+    from cochlearmodel_amd.warmstart import tol_scalars
+    _say("case: 6 tones x 60 ms, tolerance 1e-6, extrapolated guess, lockstep kernel with capture")
+    tb = _batch()
+    sc = tol_scalars(1e-6, 1)
+    v1, s1 = _run(tb, sc)
+    stride, per_item = 97, 256
+    v2, s2, rec = _capture_run(tb, sc, stride, per_item)
+    np.testing.assert_array_equal(v1, v2)
+    np.testing.assert_array_equal(s1, s2)
+    # one record per stride-th committed step of every block
+    steps = s1[:, 0]
+    expect = np.minimum(per_item, (steps - 1) // stride + 1).sum()
+    written = rec[:, 770] >= 0
+    assert written.sum() == expect
+    # records are self-consistent: rms(r_lin) and the residual block of the features
+    m = solver.section_params()[8]
+    R = rec[written & (rec[:, 770] > 0)]
+    Y, pl = R[:, 772:1028], R[:, 1028:1284]
+    # the residual in float32 with the kernel's operation order (it is a small
+    # difference of large terms, so the order matters)
+    left = np.concatenate([np.zeros((len(pl), 1), np.float32), pl[:, :-1]], 1)
+    right = np.concatenate([pl[:, 1:], np.zeros((len(pl), 1), np.float32)], 1)
+    low = np.r_[np.ones(254, np.float32), np.zeros(2, np.float32)]
+    r = Y - ((low * left + right) + m * pl)
+    rr = np.sqrt((r.astype(np.float64) ** 2).mean(1)) + 1e-30
+    np.testing.assert_allclose(R[:, 769], rr, rtol=1e-5)
+    np.testing.assert_allclose(R[:, 512:768] * R[:, 769:770], r, rtol=1e-5, atol=1e-6 * np.abs(r).max())
+    d_rms = np.sqrt((R[:, :256].astype(np.float64) ** 2).mean(1))
+    assert np.all(np.abs(d_rms - 1.0) < 1e-4)
+
+
+def test_zero_network_matches_extrapolated_guess():
+    _cuda()
+    from cochlearmodel_amd import warmstart
+    from cochlearmodel_amd.mlp import SIZES
+    _say("case: zero network, warm_start=2 vs warm_start=1")
+    tb = _batch(n=4, dur=0.04, seed=6)
+    v1, s1 = _run(tb, warmstart.tol_scalars(1e-6, 1))
+    zeros = []
+    for kin, kout in zip(SIZES[:-1], SIZES[1:]):
+        zeros += [np.zeros((kin, kout), np.float32), np.zeros(kout, np.float32)]
+    warmstart.upload(zeros, 1.0)
+    v2, s2 = _run(tb, warmstart.tol_scalars(1e-6, 2))
+    np.testing.assert_array_equal(v1, v2)
+    np.testing.assert_array_equal(s1, s2)
+
+
+def test_exact_targets_match_dense_solve():
+    torch = _cuda()
+    from cochlearmodel_amd import solver, warmstart
+    _say("case: exact targets of captured records against a dense FP64 solve")
+    tb = _batch(n=2, dur=0.04, seed=7)
+    rec = warmstart.capture(tb, 1e-6, 211, 64)
+    T = warmstart.targets(tb.bm.sec, rec, 2.0).cpu().numpy().astype(np.float64)
+    R = rec.cpu().numpy()
+    A = _matrix(solver.section_params())
+    pstar = np.linalg.solve(A, R[:, 772:1028].astype(np.float64).T).T
+    ref = (pstar - R[:, 1028:1284]) / (R[:, 769:770].astype(np.float64) * 2.0)
+    assert np.linalg.norm(T - ref) <= 1e-5 * np.linalg.norm(ref)
+
+
+def test_in_solver_network_matches_oracle():
+    """The MFMA inference inside the solver (fragment packing, tile mapping,
+    GELU, bf16 operands) against the numpy oracle on the captured inputs."""
+    _cuda()
+    from cochlearmodel_amd import warmstart
+    from cochlearmodel_amd.mlp import SIZES
+    _say("case: random network inside the solver, outputs captured")
+    rng = np.random.default_rng(11)
+    params = mo.init_params(SIZES, seed=3)
+    params[1] = (0.1 * rng.standard_normal(SIZES[1])).astype(np.float32)
+    params[3] = (0.1 * rng.standard_normal(SIZES[2])).astype(np.float32)
+    params[5] = (0.1 * rng.standard_normal(SIZES[3])).astype(np.float32)
+    warmstart.upload(params, 1.0)
+    tb = _batch(n=4, dur=0.04, seed=8)
+    _, _, rec = _capture_run(tb, warmstart.tol_scalars(1e-6, 2), 131, 64)
+    R = rec[rec[:, 770] > 0]
+    assert len(R) > 50
+    y = R[:, 1288:1544].astype(np.float64)
+    ref, _ = mo.forward(params, R[:, :769], bf16=True)
+    rel = np.linalg.norm(y - ref) / np.linalg.norm(ref)
+    _say(f"in-solver network vs oracle: rel L2 {rel:.2e} over {len(R)} records")
+    assert rel <= 1e-2
+
+
+def test_trained_network_gains_over_plain_and_extrapolated(oracle):
+    """The committed network on held-out tones: fewer Jacobi sweeps per step
+    than both guesses without it, at the same accuracy against the exact
+    per-pass solve."""
+    torch = _cuda()
+    import os
+    from cochlearmodel_amd import warmstart
+    if not os.path.exists(warmstart.MODEL_PATH):
+        pytest.skip("no trained model committed")
+    params, c_out, meta = warmstart.load_model()
+    warmstart.upload(params, c_out)
+    _say("case: trained network, 8 held-out tones x 100 ms")
+    tb = _batch(n=8, dur=0.1, seed=warmstart.EVAL_SEED + 1000)
+    res = {}
+    for name, ws in (("plain", 0), ("extrapolated", 1), ("learned", 2)):
+        v, st = _run(tb, warmstart.tol_scalars(1e-6, ws))
+        res[name] = (v, st[:, 2].sum() / st[:, 0].sum(), st[:, 0].sum())
+        _say(f"{name}: {res[name][1]:.2f} sweeps/step")
+    sel = tb.sample_blocks(8)
+    it = np.zeros(len(sel), oracle.ITEM_DTYPE)
+    for f in ("start", "mosi", "mesi", "write_start", "tol", "m1tol", "set"):
+        it[f] = tb.items[f][sel]
+    so = oracle.default_scalars(FS)
+    so.jacobi_mode = 2
+    o = oracle.bm_run(so, oracle.section_params(), np.full(256, 0.5, np.float32), it, tb.d_input.cpu().numpy(),
+                      tb.bufP1, "f32", threads=8)
+    rows = tb.block_rows(sel)
+    ex = o["speeds"][rows]
+    err = {k: np.linalg.norm(r[0][rows] - ex) / np.linalg.norm(ex) for k, r in res.items()}
+    _say(f"rel L2 vs exact solve: {err}")
+    assert res["learned"][1] < res["extrapolated"][1] < res["plain"][1]
+    assert err["learned"] <= max(1.5 * err["plain"], 1e-3)
+    assert abs(int(res["learned"][2]) - int(res["plain"][2])) <= 0.01 * res["plain"][2]
