@@ -503,7 +503,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(3)))
 // BF16 weights streamed from L2 in MFMA fragment order.  Each wave then takes
 // the plain, extrapolated or learned guess with the smallest residual and
 // iterates Jacobi to tolerance as in the fused kernel.
-constexpr int WS_XS = 792;    // LDS row stride (bf16) of the feature tile (K = 784 used)
+constexpr int WS_XS = 808;    // LDS row stride (bf16) of the feature tile (K = 800 read, 769 used)
 constexpr int WS_HS = 520;    // row stride (bf16) of hidden tiles
 constexpr int WS_OS = 260;    // row stride (float) of the output tile
 
@@ -513,56 +513,75 @@ typedef __attribute__((__vector_size__(4 * sizeof(float)))) float ws_f32x4;
 // One dense layer: out[16][N] = act(A[16][K] W + b) with NT = N/16 column tiles
 // shared out among the G waves (up to three tiles per wave, sharing the A
 // fragment of each k step).
-// Up to three column tiles (t1, t2 < 0: absent) of one layer, sharing the A
-// fragment of each k step.
-template <int KT, int NT, bool GELU>
-__device__ __forceinline__ void ws_tiles(const uint16_t *A, int sa, const uint2 *__restrict__ W,
-                                         const float *__restrict__ bias, void *out, int so, int lane, int t0, int t1,
-                                         int t2) {
-  const bool v1 = t1 >= 0, v2 = t2 >= 0;
-  ws_f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = acc0, acc2 = acc0;
-  const uint16_t *arow = A + (lane & 15) * sa + 4 * (lane >> 4);
-#pragma unroll 4
-  for (int kt = 0; kt < KT; ++kt) {
-    const uint2 av = *reinterpret_cast<const uint2 *>(arow + kt * 16);
-    const ws_bf16x4 af = __builtin_bit_cast(ws_bf16x4, av);
-    const uint2 *wk = W + (size_t)kt * NT * 64 + lane;
-    const uint2 b0 = wk[(size_t)t0 * 64];
-    acc0 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, __builtin_bit_cast(ws_bf16x4, b0), acc0, 0, 0, 0);
-    if (v1) {
-      const uint2 b1 = wk[(size_t)t1 * 64];
-      acc1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, __builtin_bit_cast(ws_bf16x4, b1), acc1, 0, 0, 0);
+#define CM_LDS __attribute__((address_space(3)))
+#define CM_GLB __attribute__((address_space(1)))
+typedef uint32_t ws_u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t ws_u32x2 __attribute__((ext_vector_type(2)));
+
+// NTL (1..3) column tiles t0, t0 + G, t0 + 2G of one layer, sharing the A
+// fragments of each k step.  Weights come as 16-byte pairs of k tiles (uint4 =
+// fragments of k tiles 2j and 2j+1); the unrolled loop keeps several pairs of
+// every tile in flight.
+template <int G, int KT, int NT, bool GELU, int NTL>
+__device__ __forceinline__ void ws_tiles(const CM_LDS uint16_t *A, int sa, const CM_GLB ws_u32x4 *W,
+                                         const CM_GLB float *bias, CM_LDS void *out, int so, int lane, int t0) {
+  constexpr int KP = (KT + 1) / 2;
+  ws_f32x4 acc[NTL];
+  UNROLL for (int i = 0; i < NTL; ++i) acc[i] = ws_f32x4{0.f, 0.f, 0.f, 0.f};
+  const CM_GLB ws_u32x4 *wl = W + t0 * 64 + lane;
+  const CM_LDS uint16_t *arow = A + (lane & 15) * sa + 4 * (lane >> 4);
+  // ring of D k-tile pairs in flight per tile: pair kp + D is requested as
+  // soon as pair kp has been consumed
+  constexpr int D = NTL == 3 ? 5 : (NTL == 2 ? 7 : 10);
+  ws_u32x4 ring[D][NTL];
+  UNROLL for (int j = 0; j < D; ++j)
+    if (j < KP) {
+      UNROLL for (int i = 0; i < NTL; ++i) ring[j][i] = wl[(j * NT + i * G) * 64];
     }
-    if (v2) {
-      const uint2 b2 = wk[(size_t)t2 * 64];
-      acc2 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, __builtin_bit_cast(ws_bf16x4, b2), acc2, 0, 0, 0);
+  __builtin_amdgcn_sched_barrier(0);
+  UNROLL for (int kp = 0; kp < KP; ++kp) {
+    const int slot = kp % D;
+    const ws_bf16x4 a0 = __builtin_bit_cast(ws_bf16x4, *reinterpret_cast<const CM_LDS ws_u32x2 *>(arow + 32 * kp));
+    const ws_bf16x4 a1 =
+        __builtin_bit_cast(ws_bf16x4, *reinterpret_cast<const CM_LDS ws_u32x2 *>(arow + 32 * kp + 16));
+    UNROLL for (int i = 0; i < NTL; ++i) {
+      const ws_u32x2 lo = ring[slot][i].xy, hi = ring[slot][i].zw;
+      acc[i] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a0, __builtin_bit_cast(ws_bf16x4, lo), acc[i], 0, 0, 0);
+      acc[i] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1, __builtin_bit_cast(ws_bf16x4, hi), acc[i], 0, 0, 0);
     }
+    if (kp + D < KP) {
+      UNROLL for (int i = 0; i < NTL; ++i) ring[slot][i] = wl[((kp + D) * NT + i * G) * 64];
+    }
+    __builtin_amdgcn_sched_barrier(0);   // keep the requests D pairs ahead of their use
   }
   // lane holds rows 4*(lane/16)+q of column 16*t + lane%16
   const int r0 = 4 * (lane >> 4), cl = lane & 15;
-  auto store = [&](const ws_f32x4 &acc, int t) {
-    const int col = 16 * t + cl;
+  UNROLL for (int i = 0; i < NTL; ++i) {
+    const int col = 16 * (t0 + i * G) + cl;
     const float bv = bias[col];
     UNROLL for (int q = 0; q < 4; ++q) {
-      const float v = acc[q] + bv;
+      const float v = acc[i][q] + bv;
       if (GELU)
-        reinterpret_cast<uint16_t *>(out)[(r0 + q) * so + col] = cm_f32_to_bf16(cm_gelu(v));
+        reinterpret_cast<CM_LDS uint16_t *>(out)[(r0 + q) * so + col] = cm_f32_to_bf16(cm_gelu(v));
       else
-        reinterpret_cast<float *>(out)[(r0 + q) * so + col] = v;
+        reinterpret_cast<CM_LDS float *>(out)[(r0 + q) * so + col] = v;
     }
-  };
-  store(acc0, t0);
-  if (v1) store(acc1, t1);
-  if (v2) store(acc2, t2);
+  }
 }
 
+// One dense layer out[16][N] = act(A[16][K] W + b), N = 16 NT: wave w computes
+// column tiles w, w + G, w + 2G, then w + 3G, ... three at a time.
 template <int G, int KT, int NT, bool GELU>
-__device__ __noinline__ void ws_layer(const uint16_t *A, int sa, const uint2 *__restrict__ W,
-                                         const float *__restrict__ bias, void *out, int so, int wave, int lane) {
-  // wave w computes tiles w, w + G, w + 2G, then w + 3G, ... three at a time
+__device__ __noinline__ void ws_layer(const CM_LDS uint16_t *A, int sa, const CM_GLB ws_u32x4 *W,
+                                      const CM_GLB float *bias, CM_LDS void *out, int so, int wave, int lane) {
   for (int t0 = wave; t0 < NT; t0 += 3 * G) {
-    const int t1 = t0 + G, t2 = t0 + 2 * G;
-    ws_tiles<KT, NT, GELU>(A, sa, W, bias, out, so, lane, t0, t1 < NT ? t1 : -1, t2 < NT ? t2 : -1);
+    const int n = t0 + 2 * G < NT ? 3 : (t0 + G < NT ? 2 : 1);
+    if (n == 3)
+      ws_tiles<G, KT, NT, GELU, 3>(A, sa, W, bias, out, so, lane, t0);
+    else if (n == 2)
+      ws_tiles<G, KT, NT, GELU, 2>(A, sa, W, bias, out, so, lane, t0);
+    else
+      ws_tiles<G, KT, NT, GELU, 1>(A, sa, W, bias, out, so, lane, t0);
   }
 }
 
@@ -657,11 +676,15 @@ __global__ __launch_bounds__(64 * G) void bm_ws_kernel(BmLaunch a, WsArgs w) {
     if (!any) break;
     if (w.use_net) {
       // X -> H2 -> H1 -> output tile in H2's space (H2 is dead once layer 2 ends)
-      ws_layer<G, 49, 32, true>(Xs, WS_XS, w.w1, w.b1, H2s, WS_HS, wave, lane);
+      typedef const CM_LDS uint16_t *lds_in;
+      typedef CM_LDS void *lds_out;
+      typedef const CM_GLB ws_u32x4 *glb_w;
+      typedef const CM_GLB float *glb_b;
+      ws_layer<G, 49, 32, true>((lds_in)Xs, WS_XS, (glb_w)w.w1, (glb_b)w.b1, (lds_out)H2s, WS_HS, wave, lane);
       __syncthreads();
-      ws_layer<G, 32, 32, true>(H2s, WS_HS, w.w2, w.b2, H1s, WS_HS, wave, lane);
+      ws_layer<G, 32, 32, true>((lds_in)H2s, WS_HS, (glb_w)w.w2, (glb_b)w.b2, (lds_out)H1s, WS_HS, wave, lane);
       __syncthreads();
-      ws_layer<G, 32, 16, false>(H1s, WS_HS, w.w3, w.b3, H2s, WS_OS, wave, lane);
+      ws_layer<G, 32, 16, false>((lds_in)H1s, WS_HS, (glb_w)w.w3, (glb_b)w.b3, (lds_out)H2s, WS_OS, wave, lane);
       __syncthreads();
     }
     // The section constants are re-read from L1/L2 here, so that they are not

@@ -33,9 +33,11 @@ struct BmLaunch {
 // Learned warm start of the predictor-pass pressure solve (warm_start = 2) and
 // the capture of its training records (capture != 0, warm_start = 1 guesses).
 struct WsArgs {
-  const uint2 *__restrict__ w1;      // MFMA-packed bf16 [49][32][64] x 4 (K 769 padded to 784)
-  const uint2 *__restrict__ w2;      // [32][32][64] x 4
-  const uint2 *__restrict__ w3;      // [32][16][64] x 4
+  // MFMA-packed bf16 weights: [k-tile pairs][n tiles][64 lanes] x 8 (lane l of
+  // pair j, tile t: W[32 j + 4 (l/16) + q][16 t + l%16] then the same for k tile 2j+1)
+  const uint4 *__restrict__ w1;      // [25][32][64] (K 769 padded to 800)
+  const uint4 *__restrict__ w2;      // [16][32][64]
+  const uint4 *__restrict__ w3;      // [16][16][64]
   const float *__restrict__ b1, *__restrict__ b2, *__restrict__ b3;
   float c_out;                       // target scale: p = p_lin + rms(r_lin) * c_out * S(y)
   int32_t use_net;                   // 1: evaluate the network and offer its guess
