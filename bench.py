@@ -52,11 +52,21 @@ def variant_scalars(name: str):
     if name == "reference_20_2":
         return sc
     sc.jacobi_mode, sc.tol_rel, sc.check_every, sc.max_sweeps = 1, TOL, 4, 1000
-    sc.warm_start = {"plain_tol": 0, "extrapolated_tol": 1}[name]
+    sc.warm_start = {"plain_tol": 0, "extrapolated_tol": 1, "learned_tol": 2}[name]
     return sc
 
 
-VARIANTS = ("reference_20_2", "plain_tol", "extrapolated_tol")
+VARIANTS = ("reference_20_2", "plain_tol", "extrapolated_tol", "learned_tol")
+MLP_FLOP_PER_STEP = 2.0 * (769 * 512 + 512 * 512 + 512 * 256)   # forward, one block-step
+PEAK_BF16_TFLOPS = 2500.0    # MI355X dense BF16 MFMA
+
+
+def load_warm_start_model():
+    """Upload the committed warm-start network into the solver."""
+    from cochlearmodel_amd import warmstart
+    params, c_out, meta = warmstart.load_model()
+    warmstart.upload(params, c_out)
+    return meta
 
 
 def sample_blocks(work, n_blocks):
@@ -114,7 +124,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--signals", type=int, default=256)
-    ap.add_argument("--variant", default="extrapolated_tol", choices=VARIANTS)
+    ap.add_argument("--variant", default="learned_tol", choices=VARIANTS)
     ap.add_argument("--cpu-blocks", type=int, default=512)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--accuracy-blocks", type=int, default=64)
@@ -128,6 +138,7 @@ def main():
     import torch
     torch.cuda.set_device(0)
     work = c2_workload(args.signals)
+    model_meta = load_warm_start_model()
     sc = variant_scalars(args.variant)
     print(f"# workload ready: {work.n} signals, {work.TB} time blocks, variant {args.variant}", file=sys.stderr,
           flush=True)
@@ -156,7 +167,10 @@ def main():
     desc = {"reference_20_2": "reference kernel-17 method (fixed 20/2 Jacobi sweeps)",
             "plain_tol": f"Jacobi to relative residual {TOL:g}, previous-pressure guess",
             "extrapolated_tol": f"Jacobi to relative residual {TOL:g}, extrapolated predictor-pass guess with "
-                                "residual gate"}
+                                "residual gate",
+            "learned_tol": f"Jacobi to relative residual {TOL:g}, learned predictor-pass guess (769-512-512-256 "
+                           "GELU MLP, BF16 MFMA in the solver) with residual gate against the previous and "
+                           "extrapolated pressure"}
     out = {
         "metric": "section-steps/s + Jacobi iters/step: learned warm start vs plain @1e-4 rel err",
         "value": value, "unit": "section-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -177,6 +191,13 @@ def main():
                                 "peak": PEAK_HBM_GBS, "unit": "GB/s",
                                 "frac": ihc_bytes / (np.mean(ihc_ms) * 1e-3) / 1e9 / PEAK_HBM_GBS},
     }
+    if args.variant == "learned_tol":
+        out["roofline"]["kernel"] = "bm_ws_kernel"
+        mlp = steps * MLP_FLOP_PER_STEP / (bm_avg * 1e-3) / 1e12
+        out["mlp_roofline"] = {"bound": "mfma", "engine": "BF16 MFMA", "achieved": mlp, "peak": PEAK_BF16_TFLOPS,
+                               "unit": "TFLOP/s", "frac": mlp / PEAK_BF16_TFLOPS,
+                               "flop_per_launch": steps * MLP_FLOP_PER_STEP}
+        out["warm_start_model"] = {k: model_meta[k] for k in ("n_signals", "records", "epochs", "final_mse")}
     threads = min(args.cpu_threads, os.cpu_count() or 1)
     if not args.no_variants:
         # every solver variant on the same resident input: BM kernel time, work and accuracy
@@ -195,14 +216,29 @@ def main():
                          "rel_l2_vs_exact_solve": float(np.linalg.norm(v - exact) / nrm)}
             print(f"# variant {name}: {ms:.1f} ms, {sw / st:.2f} sweeps/step", file=sys.stderr, flush=True)
         out["variants"] = var
+        lp, pl = var["learned_tol"], var["plain_tol"]
+        out["learned_vs_plain"] = {
+            "section_steps_per_s_ratio": lp["bm_section_steps_per_s"] / pl["bm_section_steps_per_s"],
+            "jacobi_iters_per_step_ratio": lp["jacobi_iters_per_step"] / pl["jacobi_iters_per_step"],
+            "rel_l2_vs_exact_solve": {"learned": lp["rel_l2_vs_exact_solve"], "plain": pl["rel_l2_vs_exact_solve"]}}
         out["accuracy_sample"] = (f"BM velocity of {args.accuracy_blocks} sampled time blocks ({len(rows)} rows) "
                                   "against the CPU oracle's exact per-pass pressure solve (FP32 state)")
     if not args.no_cpu:
-        cb, o, sel = cpu_baseline(work, args.cpu_blocks, threads, sc)
+        # the oracle has no network: for the learned variant the CPU baseline is
+        # the same solver with the extrapolated guess, checked bit-exact against
+        # the GPU's extrapolated run
+        cpu_variant = "extrapolated_tol" if args.variant == "learned_tol" else args.variant
+        csc = variant_scalars(cpu_variant)
+        gpu_speeds = head_speeds
+        if cpu_variant != args.variant:
+            work.run_bm(csc, False)
+            torch.cuda.synchronize()
+            gpu_speeds = work.d_speeds.cpu().numpy()
+        cb, o, sel = cpu_baseline(work, args.cpu_blocks, threads, csc)
+        cb["solver"] = cpu_variant
         out["cpu_baseline"] = cb
-        # the GPU speeds of the sampled blocks agree with the oracle
         rows = block_rows(work, sel)
-        d = np.linalg.norm(head_speeds[rows] - o["speeds"][rows]) / max(np.linalg.norm(o["speeds"][rows]), 1e-30)
+        d = np.linalg.norm(gpu_speeds[rows] - o["speeds"][rows]) / max(np.linalg.norm(o["speeds"][rows]), 1e-30)
         out["cpu_baseline"]["parity_rel_l2"] = float(d)
     print(json.dumps(out))
 
