@@ -492,7 +492,8 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(3)))
   const int wave = threadIdx.x >> 6;
   const int idx = blockIdx.x * WAVES + wave;
   __shared__ float4 pold_lds[WS ? WAVES * 65 : 1];
-  if (idx < a.n_items) run_item<TOL, WS>(a, idx, lane, pold_lds + (WS ? wave * 65 : 0));
+  if (idx < a.n_items)
+    run_item<TOL, WS>(a, a.order ? a.order[idx] : idx, lane, pold_lds + (WS ? wave * 65 : 0));
 }
 
 // ------------------------------------------- learned warm start ---------
@@ -599,8 +600,9 @@ __global__ __launch_bounds__(64 * G) void bm_ws_kernel(BmLaunch a, WsArgs w) {
 
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int idx = blockIdx.x * G + wave;
-  const bool valid = idx < a.n_items;
+  const int slot = blockIdx.x * G + wave;
+  const bool valid = slot < a.n_items;
+  const int idx = valid ? (a.order ? a.order[slot] : slot) : slot;   // item integrated by this wave
   float4 *pold = pold_lds + wave * 65;
 
   // zero the feature tile once (rows >= G stay zero)

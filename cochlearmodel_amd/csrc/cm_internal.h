@@ -18,6 +18,7 @@ struct BmLaunch {
   const float *__restrict__ sec;     // 8 x 256
   const float *__restrict__ gamma;   // sets x 256
   const cm_bm_item *__restrict__ items;
+  const int32_t *__restrict__ order;  // optional launch order: wave w integrates items[order[w]]
   int32_t n_items;
   int32_t use_queue;
   int32_t *queue;                    // work counter (use_queue != 0)

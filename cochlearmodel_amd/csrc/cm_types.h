@@ -65,6 +65,16 @@ typedef struct {
   int64_t sweeps;  // Jacobi sweeps
 } cm_item_stats;
 
+// Launch configuration chosen by cm_bm_autotune (all candidates give identical
+// results; only the time differs).
+typedef struct {
+  int32_t waves_per_block;   // 1, 2 or 4 time blocks per workgroup
+  int32_t order_lpt;         // 1: items launched longest first
+  int32_t n_trial_items;     // items in the timed sample
+  int32_t pad;
+  float ms[6];               // trial times: (wpb 1, 2, 4) x (natural, longest first)
+} cm_bm_tuning;
+
 // A run of rows of the IHC/lambda stage sharing one FIR window rule:
 // k = kbase + (row - row0); taps used = min((k+1) mod per, taps) with
 // per = L when k < frdp-1 and per = tla afterwards (src/cochlea.cu:2719-2728).

@@ -57,8 +57,10 @@ struct BMHolder {
   DevBuf<float> conv_time, conv_loops, conv_jacobi;
   DevBuf<float> conv_time_blocks, conv_loops_blocks, conv_jacobi_blocks;
   DevBuf<float> tol, m1tol;   // per-block tolerances (setupToleranceProfile)
-  DevBuf<cm_bm_item> items;
+  DevBuf<cm_bm_item> items, trial_items;
   DevBuf<cm_item_stats> stats;
+  DevBuf<int32_t> order, trial_order;   // launch order (cm_set_bm_order) and autotune scratch
+  int order_n = 0;
   size_t input_nodes = 0, result_nodes = 0, lambda_nodes = 0;
   int n_sets = 1;
   int tol_blocks = 0;
@@ -139,7 +141,8 @@ void block_decomposition(int TB, int bufP1, int overlap, int DF, std::vector<cm_
 int launch_bm_items(const cm_bm_scalars &sc, const float *d_input, int64_t input_len, const float *d_sec,
                     const float *d_gamma, const cm_bm_item *d_items, int n_items, float *d_speeds, int64_t rows,
                     int32_t *failed_nodes, float *ct, float *cl, float *cj, int32_t *fb, float *ctb, float *clb,
-                    float *cjb, cm_item_stats *d_stats, int waves_per_block, int persistent, float *kernel_ms) {
+                    float *cjb, cm_item_stats *d_stats, int waves_per_block, int persistent, float *kernel_ms,
+                    const int32_t *order_override) {
   BmLaunch a;
   a.sc = sc;
   a.input = d_input;
@@ -159,6 +162,16 @@ int launch_bm_items(const cm_bm_scalars &sc, const float *d_input, int64_t input
   a.conv_loops_blocks = clb;
   a.conv_jacobi_blocks = cjb;
   a.stats = d_stats;
+  a.order = nullptr;
+  if (g_bm.order_n > 0) {
+    if (g_bm.order_n != n_items) {
+      set_error("launch order holds %d items, the launch has %d (cm_set_bm_order(NULL, 0) clears it)",
+                g_bm.order_n, n_items);
+      return -3;
+    }
+    a.order = g_bm.order.ptr;
+  }
+  if (order_override) a.order = order_override;
   a.use_queue = persistent ? 1 : 0;
   a.queue = nullptr;
   int grid;
@@ -214,6 +227,100 @@ int cm_set_stream(void *s) {
 int cm_device_synchronize(void) { return check(hipDeviceSynchronize(), "hipDeviceSynchronize"); }
 
 float cm_last_kernel_ms(void) { return g_bm.last_kernel_ms; }
+
+// Host-side longest-first order of a block table (stable for equal lengths).
+static void lpt_order(const cm_bm_item *items, int n, std::vector<int32_t> &ord) {
+  ord.resize(n);
+  for (int i = 0; i < n; ++i) ord[i] = i;
+  std::stable_sort(ord.begin(), ord.end(), [&](int32_t x, int32_t y) {
+    return (items[x].mesi - items[x].start) > (items[y].mesi - items[y].start);
+  });
+}
+
+int cm_set_bm_order(const int32_t *h_order, int n) {
+  if (!h_order || n <= 0) {
+    g_bm.order_n = 0;
+    return 0;
+  }
+  std::vector<char> seen(n, 0);
+  for (int i = 0; i < n; ++i) {
+    if (h_order[i] < 0 || h_order[i] >= n || seen[h_order[i]]) {
+      set_error("cm_set_bm_order: not a permutation of 0..%d (entry %d)", n - 1, i);
+      return -3;
+    }
+    seen[h_order[i]] = 1;
+  }
+  if (g_bm.order.ensure(n)) return -1;
+  if (check(hipMemcpy(g_bm.order.ptr, h_order, n * sizeof(int32_t), hipMemcpyHostToDevice), "upload order"))
+    return -1;
+  g_bm.order_n = n;
+  return 0;
+}
+
+// Times every launch configuration on a sample of the block table (every
+// trial_stride-th block in longest-first order, so the sample keeps the length
+// mix) and returns the fastest; with order_lpt the full longest-first order is
+// installed for the following launches of this table.  Trial runs write their
+// rows of d_speeds, which the real launch overwrites.
+int cm_bm_autotune(const cm_bm_scalars *sc, const float *d_input, int64_t input_len, const float *d_sec,
+                   const float *d_gamma, const cm_bm_item *h_items, int n_items, float *d_speeds, int64_t rows,
+                   int trial_stride, cm_bm_tuning *out) {
+  if (!sc || !d_input || !d_sec || !d_gamma || !h_items || n_items <= 0 || !d_speeds || !out || trial_stride < 1) {
+    set_error("cm_bm_autotune: bad arguments");
+    return -3;
+  }
+  std::vector<int32_t> full;
+  lpt_order(h_items, n_items, full);
+  std::vector<cm_bm_item> sample;
+  for (int i = 0; i < n_items; i += trial_stride) sample.push_back(h_items[full[i]]);
+  // natural order of the sample = the blocks' original order
+  std::vector<int32_t> pick;
+  for (int i = 0; i < n_items; i += trial_stride) pick.push_back(full[i]);
+  std::vector<int32_t> nat(pick.size());
+  for (size_t i = 0; i < nat.size(); ++i) nat[i] = (int32_t)i;
+  std::stable_sort(nat.begin(), nat.end(), [&](int32_t x, int32_t y) { return pick[x] < pick[y]; });
+  std::vector<int32_t> lpt(sample.size());
+  for (size_t i = 0; i < lpt.size(); ++i) lpt[i] = (int32_t)i;   // sample is already longest first
+  const int ns = (int)sample.size();
+  if (g_bm.trial_items.ensure(ns) || g_bm.trial_order.ensure(2 * ns)) return -1;
+  if (check(hipMemcpy(g_bm.trial_items.ptr, sample.data(), ns * sizeof(cm_bm_item), hipMemcpyHostToDevice),
+            "upload trial items") ||
+      check(hipMemcpy(g_bm.trial_order.ptr, nat.data(), ns * sizeof(int32_t), hipMemcpyHostToDevice),
+            "upload trial order") ||
+      check(hipMemcpy(g_bm.trial_order.ptr + ns, lpt.data(), ns * sizeof(int32_t), hipMemcpyHostToDevice),
+            "upload trial order"))
+    return -1;
+  const int saved_n = g_bm.order_n;
+  g_bm.order_n = 0;
+  const int wpb[3] = {1, 2, 4};
+  float best = 1e30f;
+  int bi = 0;
+  for (int c = -1; c < 6; ++c) {   // c = -1: untimed warm-up
+    const int w = wpb[(c < 0 ? 0 : c) / 2], o = (c < 0 ? 0 : c) % 2;
+    float ms = 0.f;
+    const int rc = launch_bm_items(*sc, d_input, input_len, d_sec, d_gamma, g_bm.trial_items.ptr, ns, d_speeds, rows,
+                                   nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                   w, 0, &ms, g_bm.trial_order.ptr + o * ns);
+    if (rc) {
+      g_bm.order_n = saved_n;
+      return rc;
+    }
+    if (c < 0) continue;
+    out->ms[c] = ms;
+    if (ms < best) {
+      best = ms;
+      bi = c;
+    }
+  }
+  g_bm.order_n = saved_n;
+  out->waves_per_block = wpb[bi / 2];
+  out->order_lpt = bi % 2;
+  out->n_trial_items = ns;
+  out->pad = 0;
+  if (out->order_lpt) return cm_set_bm_order(full.data(), n_items);
+  g_bm.order_n = 0;
+  return 0;
+}
 
 int cm_set_bm_launch(int waves_per_block, int persistent) {
   if (waves_per_block != 1 && waves_per_block != 2 && waves_per_block != 4) {
@@ -405,7 +512,7 @@ int BMOHCNewKernel(const float *input_samples, int override_input_samples, float
                            g_bm.failed_nodes.ptr, g_bm.conv_time.ptr, g_bm.conv_loops.ptr, g_bm.conv_jacobi.ptr,
                            g_bm.failed_blocks.ptr, g_bm.conv_time_blocks.ptr, g_bm.conv_loops_blocks.ptr,
                            g_bm.conv_jacobi_blocks.ptr, g_bm.stats.ptr, g_bm.waves_per_block, g_bm.persistent,
-                           (Show_Run_Time & 1) ? &ms : nullptr);
+                           (Show_Run_Time & 1) ? &ms : nullptr, nullptr);
   if (rc) return rc;
   g_bm.last_kernel_ms = ms;
   return check(hipStreamSynchronize(stream()), "BM kernel");
@@ -443,7 +550,7 @@ int cm_bm_run(const cm_bm_scalars *sc, const float *d_input, int64_t input_len, 
   int rc = launch_bm_items(*sc, d_input, input_len, d_sec, d_gamma, d_items, n_items, d_speeds, rows, d_failed_nodes,
                            d_conv_time, d_conv_loops, d_conv_jacobi, d_failed_blocks, d_conv_time_blocks,
                            d_conv_loops_blocks, d_conv_jacobi_blocks, d_stats, waves_per_block, persistent,
-                           timed ? &ms : nullptr);
+                           timed ? &ms : nullptr, nullptr);
   if (rc) return rc;
   if (timed) g_bm.last_kernel_ms = ms;
   return 0;

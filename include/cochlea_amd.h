@@ -51,6 +51,18 @@ int cm_device_synchronize(void);
 float cm_last_kernel_ms(void);
 /* BM launch shape: waves (time blocks) per workgroup, persistent queue. */
 int cm_set_bm_launch(int waves_per_block, int persistent);
+/* Launch order of the next BM launches over an n-block table: wave w
+ * integrates block h_order[w] (a permutation; results do not depend on it).
+ * NULL clears it. */
+int cm_set_bm_order(const int32_t *h_order, int n);
+/* Times the launch configurations (waves per workgroup x natural or
+ * longest-first order) on every trial_stride-th block of the table (taken in
+ * longest-first order) and reports the fastest; installs the longest-first
+ * order when it wins.  Results are identical for every configuration.
+ * h_items: the block table on the host; d_speeds receives the trial rows. */
+int cm_bm_autotune(const cm_bm_scalars *sc, const float *d_input, int64_t input_len, const float *d_sec,
+                   const float *d_gamma, const cm_bm_item *h_items, int n_items, float *d_speeds, int64_t rows,
+                   int trial_stride, cm_bm_tuning *out);
 
 /* ---- BM velocity ------------------------------------------------------- */
 /* replaces BMOHCKernel_Init, include/cochlea_common.h:209-230 */
