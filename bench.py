@@ -118,6 +118,109 @@ def cpu_baseline(work, n_blocks: int, threads: int, sc_gpu):
                       f"{dt:.2f} s wall, FP32 oracle of the same solver settings with OpenMP over blocks"}, o, sel
 
 
+def c3_main(args):
+    """Uneven batch: 1024 seeded signals (20-500 ms tones, 3-harmonic complexes
+    and noise, 0-100 dB, per-signal OHC spline and IHC), reference method,
+    launch shape from the autotuner; one step = BM + IHC/lambda."""
+    import torch
+    from cochlearmodel_amd import solver, workloads
+    from cochlearmodel_amd.uneven import UnevenBatch
+    sigs, ohc, ihc, meta = workloads.c3_signals(args.c3_signals)
+    ub = UnevenBatch(sigs, ohc, ihc, FS)
+    sc = solver.default_scalars(FS)
+    print(f"# uneven batch ready: {ub.n} signals, {ub.TB} time blocks, {ub.rows} rows", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    tune = ub.autotune(sc, trial_stride=8)
+    t_tune = time.perf_counter() - t0
+    print(f"# autotune {t_tune:.1f} s: {tune}", file=sys.stderr, flush=True)
+    d_lambda = torch.empty((3, ub.rows, SECTIONS), dtype=torch.float32, device="cuda")
+    d_scratch = torch.empty((2, ub.rows, SECTIONS), dtype=torch.float32, device="cuda")
+    for _ in range(args.warmup):
+        ub.run_bm(sc)
+        ub.run_lambda(d_lambda, d_scratch)
+    torch.cuda.synchronize()
+    bm_ms, lam_ms = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        bm_ms.append(ub.run_bm(sc))
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        ub.run_lambda(d_lambda, d_scratch)
+        e1.record()
+        e1.synchronize()
+        lam_ms.append(e0.elapsed_time(e1))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    steps, passes, sweeps = ub.stats()
+    ms_per_step = 1e3 * wall / args.steps
+    bm_avg = float(np.mean(bm_ms))
+    # the same launch without the tuning: one wave per workgroup, natural order
+    ub.clear_order()
+    ms_untuned = ub.run_bm(sc, waves_per_block=1)
+    flops = flops_per_launch(steps, passes, sweeps)
+    lam_bytes = ub.rows * SECTIONS * 16.0
+    out = {
+        "metric": "section-steps/s (uneven batch)", "value": SECTIONS * steps / (ms_per_step * 1e-3),
+        "unit": "section-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{ub.n} seeded uneven signals (20-500 ms; tones 125-10000 Hz, 3-harmonic complexes, "
+                               "Gaussian noise; 0-100 dB; per-signal OHC spline in [0, 0.5] and IHC in [5, 8]), "
+                               "44.1 kHz, 20 ms time blocks with 5 ms overlap, reference kernel-17 method, "
+                               "BM + IHC/lambda per step",
+                   "signals": ub.n, "time_blocks": ub.TB, "rows": ub.rows, "samples": int(sum(ub.lengths))},
+        "bm_ms": bm_avg, "ihc_lambda_ms": float(np.mean(lam_ms)), "committed_steps": steps,
+        "jacobi_iters_per_step": sweeps / steps, "signals_per_s": ub.n / (ms_per_step * 1e-3),
+        "autotune": {"waves_per_block": int(tune["waves_per_block"]), "longest_first": bool(tune["order_lpt"]),
+                     "trial_blocks": int(tune["n_trial_items"]), "trial_ms": [float(x) for x in tune["ms"]],
+                     "seconds": t_tune, "bm_ms_untuned": ms_untuned},
+        "roofline": {"bound": "mfma", "engine": "FP32 VALU", "achieved": flops / (bm_avg * 1e-3) / 1e12,
+                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": flops / (bm_avg * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, "traffic": None,
+                     "kernel": "bm_kernel", "flop_per_launch": flops},
+        "ihc_lambda_roofline": {"bound": "hbm", "achieved": lam_bytes / (np.mean(lam_ms) * 1e-3) / 1e9,
+                                "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                                "frac": lam_bytes / (np.mean(lam_ms) * 1e-3) / 1e9 / PEAK_HBM_GBS},
+    }
+    if not args.no_cpu:
+        from oracle import oracle as O
+        threads = min(args.cpu_threads, os.cpu_count() or 1)
+        sel = np.arange(0, ub.TB, max(1, ub.TB // args.cpu_blocks))[:args.cpu_blocks]
+        # the sampled blocks on a compacted input: block j reads only input
+        # [start, mesi + 2) and writes rows [write_start, mesi), so re-basing
+        # each block onto its own copied range leaves its arithmetic unchanged
+        it = np.zeros(len(sel), O.ITEM_DTYPE)
+        pieces, grows, crows, off = [], [], [], 0
+        for j, b in enumerate(sel):
+            item = ub.items[b]
+            a0, a1 = int(item["start"]), int(item["mesi"]) + 2
+            pieces.append(ub.input[a0:min(a1, ub.rows)])
+            if a1 > ub.rows:
+                pieces.append(np.zeros(a1 - ub.rows, np.float32))
+            shift = off - a0
+            it[j] = (a0 + shift, item["mosi"] + shift, item["mesi"] + shift, item["write_start"] + shift,
+                     item["tol"], item["m1tol"], item["set"], 0)
+            grows.append(np.arange(item["write_start"], item["mesi"]))
+            crows.append(np.arange(item["write_start"] + shift, item["mesi"] + shift))
+            off += a1 - a0
+        cin = np.concatenate(pieces)
+        t1 = time.perf_counter()
+        o = O.bm_run(O.default_scalars(FS), O.section_params(), ohc, it, cin, off, "f32", threads=threads)
+        dt = time.perf_counter() - t1
+        st = int(o["steps"].sum())
+        d_rows = torch.from_numpy(np.concatenate(grows)).to("cuda")
+        ub.run_bm(sc, timed=False)
+        g = ub.d_speeds.index_select(0, d_rows).cpu().numpy()
+        ref = o["speeds"][np.concatenate(crows)]
+        out["cpu_baseline"] = {"value": SECTIONS * st / dt, "unit": "section-steps/s", "cores": threads,
+                               "kind": "port", "solver": "reference_20_2",
+                               "sample": f"{len(sel)} of {ub.TB} time blocks (every {max(1, ub.TB // args.cpu_blocks)}"
+                                         f"th), {st} committed steps, {dt:.2f} s wall, FP32 oracle, OpenMP over "
+                                         "blocks",
+                               "parity_rel_l2": float(np.linalg.norm(g - ref) / max(np.linalg.norm(ref), 1e-30))}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -130,6 +233,11 @@ def main():
     ap.add_argument("--accuracy-blocks", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--workload", default="c2", choices=("c2", "c3"),
+                    help="c2: 256 uniform tones (headline); c3: 1024 uneven signals")
+    ap.add_argument("--c3-signals", type=int, default=1024)
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="first run one device-to-device copy of this many MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "only one GPU is used: runs on several GPUs are not supported",
@@ -137,6 +245,16 @@ def main():
         sys.exit(2)
     import torch
     torch.cuda.set_device(0)
+    if args.copy_probe_mb > 0:
+        n = args.copy_probe_mb * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+    if args.workload == "c3":
+        return c3_main(args)
     work = c2_workload(args.signals)
     model_meta = load_warm_start_model()
     sc = variant_scalars(args.variant)
