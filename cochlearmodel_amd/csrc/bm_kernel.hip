@@ -533,7 +533,7 @@ __device__ __forceinline__ void ws_tiles(const CM_LDS uint16_t *A, int sa, const
   const CM_LDS uint16_t *arow = A + (lane & 15) * sa + 4 * (lane >> 4);
   // ring of D k-tile pairs in flight per tile: pair kp + D is requested as
   // soon as pair kp has been consumed
-  constexpr int D = NTL == 3 ? 5 : (NTL == 2 ? 7 : 10);
+  constexpr int D = NTL == 3 ? 3 : (NTL == 2 ? 4 : 6);
   ws_u32x4 ring[D][NTL];
   UNROLL for (int j = 0; j < D; ++j)
     if (j < KP) {
@@ -695,9 +695,10 @@ __global__ __launch_bounds__(64 * G) void bm_ws_kernel(BmLaunch a, WsArgs w) {
     asm volatile("" ::: "memory");
     load_consts<true>(a, it, lane, k);
     if (active) {
-      // the predictor pass's right-hand side again (same inputs, same values):
-      // cheaper than holding it in registers across the network evaluation
+      // the predictor state and right-hand side again (same inputs, same
+      // values): cheaper than holding them in registers across the network
       float od[4], G4[4], Y[4];
+      pass_begin(s, true);
       const float sf = pass_rhs(k, c, s, true, lane, od, G4, Y);
       const float yy = tree_sum(Y[0] * Y[0], Y[1] * Y[1], Y[2] * Y[2], Y[3] * Y[3]);
       if (w.use_net) {
