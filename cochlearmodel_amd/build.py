@@ -18,6 +18,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcochlea_amd.so")
+DRIVER_SRC = os.path.join(HERE, "host", "cochlea_run.cpp")
+DRIVER = os.path.join(HERE, "cochlea_run")
 ROOT = os.path.dirname(HERE)
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
@@ -51,8 +53,16 @@ def build(verbose: bool = False) -> str:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("link of libcochlea_amd.so failed")
     os.replace(tmp, LIB)
+    # command-line host of the VOICE path, linked against the library
+    cmd = [HIPCC, "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), DRIVER_SRC, "-o",
+           DRIVER + ".tmp", "-L", HERE, "-lcochlea_amd", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("build of cochlea_run failed")
+    os.replace(DRIVER + ".tmp", DRIVER)
     if verbose:
-        print(f"built {LIB} from {len(sources)} sources")
+        print(f"built {LIB} from {len(sources)} sources and {DRIVER}")
     return LIB
 
 

@@ -1,0 +1,363 @@
+// cochlea_run -- command-line host of the VOICE path with the reference's
+// solver/params API, calling libcochlea_amd.so through its C-ABI.
+//
+//   cochlea_run <file.par>           run and write Output_Result
+//   cochlea_run --parse <file.par>   print the parsed key = value map
+//
+// Parameter file (ConfigFileHandler, src/ConfigFileHandler.cpp:32-56): one
+// `key = value` or `key = "value"` per line; keys are [_A-Za-z0-9]+; the first
+// occurrence of a key wins (std::map::insert); CRLF line ends are accepted.
+// Values are read like parseToScalar / parseToVector (include/cvector.cpp:403-424):
+// a stream extraction, 0 when it fails; vectors split on white space.  Lines
+// are matched one at a time (the reference's regex relies on a per-line `$`).
+//
+// Run (CSolver::Run_Cuda, src/solver.cpp:380-917, one VOICE chunk):
+// BMOHCKernel_Init -> BMOHCNewKernel -> copies -> write failed nodes/blocks,
+// convergence data per Convergence_Tests bits, BM velocity (Allowed_Outputs
+// bit 0) -> IHCNewKernel + RunIHCKernel -> lambda high/medium/low (bits 1-3)
+// -> timers.  Output file (OutputBuffer<float>, include/OutputBuffer.h:47-123):
+// the float32 concatenation of every written vector, in that order.  The
+// emitted rows of a first chunk are [overlap, totalTimeNodes) (the transient
+// overlap is discarded, src/solver.cpp:704-727).
+//
+// Supported keys: Fs, Input_Signal, Input_File (raw float64, CBin), Sin_Frequency
+// and Sin_dB (generated tone when no input is given), Duration, Time_Blocks,
+// Time_Block_Length, Overlap_Time, OHC_Vector, IHC_Vector, Allowed_Outputs,
+// Discard_BM_Velocity_Output, Discard_Lambdas_Output, Output_Result,
+// JACOBBY_Loops_Fast, JACOBBY_Loops_Slow, Cuda_Outern_Loops, MAX_M1_SP_ERROR,
+// MAX_TOLERANCE, RELATIVE_ERRORS, M1_SP_Fix_Factor, Tolerance_Fix_Factor, SPLRef,
+// Decouple_Filter, Convergence_Tests, Lambda_SAT, Disable_Lambda,
+// AC_Filter_Vector (Filter_Mode 2 vector: [order, isFIR, (a...), b...]).
+#include <algorithm>
+#include <cctype>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <map>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "cochlea_amd.h"
+
+namespace {
+
+constexpr int kSections = 256;
+constexpr int kLambdas = 3;
+
+// ------------------------------------------------------------ par file ----
+class ParFile {
+ public:
+  explicit ParFile(const std::string &path) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) throw std::runtime_error("cannot open parameter file " + path);
+    std::string line;
+    while (std::getline(f, line)) parse_line(line);
+  }
+  bool has(const std::string &k) const { return kv_.count(k) != 0; }
+  std::string str(const std::string &k, const std::string &def = "") const {
+    auto it = kv_.find(k);
+    return it == kv_.end() ? def : it->second;
+  }
+  template <class T> T scalar(const std::string &k, T def) const {
+    if (!has(k)) return def;
+    std::stringstream ss(str(k));
+    T v;
+    return (ss >> v) ? v : T(0);
+  }
+  template <class T> std::vector<T> vec(const std::string &k) const {
+    std::vector<T> out;
+    std::stringstream ss(str(k));
+    std::string tok;
+    while (ss >> tok) {
+      std::stringstream ts(tok);
+      T v;
+      out.push_back((ts >> v) ? v : T(0));
+    }
+    return out;
+  }
+  const std::map<std::string, std::string> &map() const { return kv_; }
+
+ private:
+  static bool key_char(char c) { return std::isalnum((unsigned char)c) || c == '_'; }
+  void parse_line(std::string line) {
+    while (!line.empty() && (line.back() == '\r' || line.back() == '\n')) line.pop_back();
+    // find "key\s*=\s*value" anywhere on the line (regex search semantics)
+    for (size_t eq = line.find('='); eq != std::string::npos; eq = line.find('=', eq + 1)) {
+      size_t ke = eq;
+      while (ke > 0 && std::isspace((unsigned char)line[ke - 1])) --ke;
+      size_t ks = ke;
+      while (ks > 0 && key_char(line[ks - 1])) --ks;
+      if (ks == ke) continue;
+      size_t vs = eq + 1;
+      while (vs < line.size() && std::isspace((unsigned char)line[vs])) ++vs;
+      if (vs >= line.size()) continue;
+      std::string value;
+      if (line[vs] == '"') {
+        const size_t close = line.rfind('"');
+        if (close == vs || close + 1 != line.size()) continue;   // quoted value must end the line
+        value = line.substr(vs + 1, close - vs - 1);
+      } else {
+        value = line.substr(vs);
+        if (value.find('"') != std::string::npos) continue;
+      }
+      kv_.insert({line.substr(ks, ke - ks), value});
+      return;
+    }
+  }
+  std::map<std::string, std::string> kv_;
+};
+
+// --------------------------------------------------------------- helpers ---
+void ok(int rc, const char *what) {
+  if (rc != 0) throw std::runtime_error(std::string(what) + ": " + cm_last_error());
+}
+
+// expandVectorToSize (include/cvector.cpp:609-624): each entry repeated in a block
+std::vector<float> expand(const std::vector<double> &v, int size) {
+  std::vector<float> out;
+  if (v.empty()) return std::vector<float>(size, 0.f);
+  const int n = (int)v.size(), times = size / n, rem = size % n;
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < times + (i < rem ? 1 : 0); ++j) out.push_back((float)v[i]);
+  return out;
+}
+
+std::vector<double> read_float64_file(const std::string &path) {
+  std::ifstream f(path, std::ios::binary);
+  if (!f) throw std::runtime_error("cannot open input file " + path);
+  f.seekg(0, std::ios::end);
+  const size_t n = (size_t)f.tellg() / sizeof(double);
+  f.seekg(0);
+  std::vector<double> v(n);
+  f.read(reinterpret_cast<char *>(v.data()), n * sizeof(double));
+  return v;
+}
+
+// the committed default AC filter: the numbers of "filter_mode_2": [ ... ]
+std::vector<double> read_filter_fixture(const std::string &path) {
+  std::ifstream f(path);
+  if (!f) throw std::runtime_error("cannot open AC filter fixture " + path);
+  std::stringstream ss;
+  ss << f.rdbuf();
+  const std::string s = ss.str();
+  const size_t k = s.find("\"filter_mode_2\"");
+  const size_t a = s.find('[', k), b = s.find(']', a);
+  if (k == std::string::npos || a == std::string::npos || b == std::string::npos)
+    throw std::runtime_error("no filter_mode_2 vector in " + path);
+  std::string body = s.substr(a + 1, b - a - 1);
+  for (char &c : body)
+    if (c == ',') c = ' ';
+  std::stringstream bs(body);
+  std::vector<double> v;
+  double x;
+  while (bs >> x) v.push_back(x);
+  return v;
+}
+
+class BinWriter {
+ public:
+  explicit BinWriter(const std::string &path) : f_(path, std::ios::binary) {
+    if (!f_) throw std::runtime_error("cannot open output file " + path);
+  }
+  void write(const float *v, size_t n) { f_.write(reinterpret_cast<const char *>(v), n * sizeof(float)); }
+  template <class T> void write_cast(const T *v, size_t n) {
+    std::vector<float> t(v, v + n);
+    write(t.data(), n);
+  }
+
+ private:
+  std::ofstream f_;
+};
+
+std::string dir_of(const std::string &p) {
+  const size_t s = p.find_last_of('/');
+  return s == std::string::npos ? "." : p.substr(0, s);
+}
+
+// ------------------------------------------------------------------ run ---
+int run(const ParFile &par, const std::string &exe_dir) {
+  const float Fs = par.scalar<float>("Fs", 44100.f);
+  const float block_len = par.scalar<float>("Time_Block_Length", 0.02f);
+  const double overlap_time = par.scalar<double>("Overlap_Time", 0.005);
+  const int block = (int)std::ceil(block_len * Fs);
+  const long overlap_us = (long)std::llround(overlap_time * 1000.0) * 1000;
+  const int overlap = (int)(overlap_us * (double)Fs / 1000000.0);
+
+  // input signal (Input_Signal, Input_File, or Sin_Frequency/Sin_dB)
+  std::vector<double> signal;
+  if (par.has("Input_Signal")) {
+    signal = par.vec<double>("Input_Signal");
+  } else if (par.has("Input_File") && !par.str("Input_File").empty()) {
+    signal = read_float64_file(par.str("Input_File"));
+  } else if (par.has("Sin_Frequency") && par.has("Sin_dB")) {
+    const double f = par.scalar<double>("Sin_Frequency", 1000.0), db = par.scalar<double>("Sin_dB", 60.0);
+    const double amp = 10.0 * 20e-6 * std::pow(10.0, db / 20.0);   // CONV_dB_TO_AMP, include/const.h:53
+    const int n = (int)std::lround(par.scalar<double>("Duration", 0.1) * Fs);
+    signal.resize(n);
+    for (int i = 0; i < n; ++i) signal[i] = amp * std::sin(2.0 * M_PI * f * i / Fs);
+  } else {
+    throw std::runtime_error("no input: set Input_Signal, Input_File or Sin_Frequency/Sin_dB");
+  }
+  int TB = par.scalar<int>("Time_Blocks", 0);
+  if (TB <= 0) {
+    const double duration = par.scalar<double>("Duration", (double)signal.size() / Fs);
+    TB = std::max(1, (int)std::lround(duration / block_len));
+  }
+  const int bufP1 = overlap + block * (TB + 1);
+  const int total = overlap + block * TB;
+  const size_t in_nodes = (size_t)block * (TB + 1) + 2 * overlap;
+  const size_t res_nodes = in_nodes * kSections;
+
+  unsigned allowed = 0;
+  if (!par.scalar<int>("Discard_BM_Velocity_Output", 0)) allowed |= 1;
+  if (!par.scalar<int>("Discard_Lambdas_Output", 0)) allowed |= 14;
+  allowed = par.scalar<unsigned>("Allowed_Outputs", allowed);
+  const int conv_tests = par.scalar<int>("Convergence_Tests", 0);
+  const std::string out_path = par.str("Output_Result");
+  if (out_path.empty()) throw std::runtime_error("Output_Result is not set");
+
+  // model (CModel, src/model.cpp:18-86; Init_Tri_Matrix, src/solver.cpp:155-176)
+  std::vector<float> sec(9 * kSections);
+  ok(cm_pack_section_params(sec.data()), "cm_pack_section_params");
+  std::vector<double> lo(kSections), mid(kSections), up(kSections);
+  ok(cm_tri_matrix(lo.data(), mid.data(), up.data()), "cm_tri_matrix");
+  std::vector<float> upf(up.begin(), up.end()), lof(lo.begin(), lo.end());
+  std::vector<double> ohc = par.has("OHC_Vector") ? par.vec<double>("OHC_Vector") : std::vector<double>{0.5};
+  const std::vector<float> gamma = expand(ohc, kSections);
+  const float *S = sec.data();
+  ok(BMOHCKernel_Init(gamma.data(), S + 0 * kSections, S + 1 * kSections, upf.data(), lof.data(), S + 2 * kSections,
+                      S + 3 * kSections, S + 4 * kSections, S + 5 * kSections, S + 6 * kSections, S + 7 * kSections,
+                      1.f, 0.f, in_nodes, res_nodes, res_nodes * 3, 1, 0, 0),
+     "BMOHCKernel_Init");
+
+  std::vector<float> input(bufP1, 0.f);
+  for (int i = 0; i < bufP1 && i < (int)signal.size(); ++i) input[i] = (float)signal[i];
+  cm_bm_scalars sc;
+  ok(cm_default_bm_scalars(Fs, &sc), "cm_default_bm_scalars");
+  const int j_fast = par.scalar<int>("JACOBBY_Loops_Fast", 2), j_slow = par.scalar<int>("JACOBBY_Loops_Slow", 20);
+  const int passes = par.scalar<int>("Cuda_Outern_Loops", 10);
+  const int DF = par.scalar<int>("Decouple_Filter", 0);
+  ok(BMOHCNewKernel(input.data(), 0, sc.w_ohc, sc.time_step, sc.time_step_out, sc.delta_x, sc.alpha_r, 1, 1, 0, sc.Ts,
+                    sc.alpha_l, sc.alpha_s, sc.Gme, sc.a0, sc.a1, sc.a2, sc.sigma_ow, sc.eta1, sc.eta2, TB, bufP1,
+                    overlap, std::lround(overlap / (double)Fs * 1e6), 1, sc.dt_max, sc.dt_min, DF,
+                    par.scalar<float>("MAX_M1_SP_ERROR", 1e-15f), par.scalar<float>("MAX_TOLERANCE", 1e-10f),
+                    par.scalar<int>("RELATIVE_ERRORS", 0), 1, par.scalar<float>("M1_SP_Fix_Factor", 0.f),
+                    par.scalar<float>("Tolerance_Fix_Factor", 0.f), par.scalar<float>("SPLRef", 2e-5f), 0, 0, 1,
+                    j_fast, j_slow, passes, 17, 2),
+     "BMOHCNewKernel");
+  const float bm_ms = cm_last_kernel_ms();
+
+  std::vector<float> speeds((size_t)bufP1 * kSections);
+  ok(BMOHCKernel_Copy_Results(speeds.data(), speeds.size(), 0), "BMOHCKernel_Copy_Results");
+  auto dev_copy = [&](void *src, size_t n_bytes, void *dst) {
+    ok(GeneralKernel_Copy_Results_Bytes(dst, src, n_bytes, 0), "GeneralKernel_Copy_Results_Bytes");
+  };
+  std::vector<int32_t> failed_nodes(bufP1), failed_blocks(TB);
+  dev_copy(getCudaFailedTimeNodes(), failed_nodes.size() * 4, failed_nodes.data());
+  dev_copy(getCudaFailedBlocks(), failed_blocks.size() * 4, failed_blocks.data());
+  std::vector<float> conv_times(bufP1), conv_times_b(TB), conv_loops(bufP1), conv_loops_b(TB), conv_j(bufP1),
+      conv_j_b(TB);
+  ok(extractConvergenceTimes(conv_times.data(), conv_times.size()), "extractConvergenceTimes");
+  dev_copy(getCudaConvergeSpeedBlocks(), TB * 4, conv_times_b.data());
+  dev_copy(getCudaConvergedTimeNodes(), bufP1 * 4, conv_loops.data());
+  dev_copy(getCudaConvergedBlocks(), TB * 4, conv_loops_b.data());
+  dev_copy(getCudaConvergedJacobyLoopsPerIteration(), bufP1 * 4, conv_j.data());
+  dev_copy(getCudaConvergedJacobyLoopsPerIterationBlocks(), TB * 4, conv_j_b.data());
+
+  BinWriter out(out_path);
+  const size_t rows = (size_t)(total - overlap);
+  if (conv_tests & 1) out.write(conv_times.data() + overlap, rows);
+  if (conv_tests & 2) out.write(conv_times_b.data(), TB);
+  out.write_cast(failed_nodes.data() + overlap, rows);
+  out.write_cast(failed_blocks.data(), TB);
+  if (conv_tests & 4) {
+    out.write(conv_loops.data() + overlap, rows);
+    out.write(conv_loops_b.data(), TB);
+  }
+  if (conv_tests & 8) {
+    out.write(conv_j.data() + overlap, rows);
+    out.write(conv_j_b.data(), TB);
+  }
+  if (allowed & 1) out.write(speeds.data() + (size_t)overlap * kSections, rows * kSections);
+
+  float ihc_ms = 0.f;
+  if (!par.scalar<int>("Disable_Lambda", 0) && (allowed & 14)) {
+    // AudioGramCreator::setupIHCOHC (src/AudioGramCreator.cpp:603-823)
+    std::vector<double> fm2 = par.has("AC_Filter_Vector")
+                                  ? par.vec<double>("AC_Filter_Vector")
+                                  : read_filter_fixture(exe_dir + "/data/ac_filter_equiripple_" +
+                                                        std::to_string((int)Fs) + ".json");
+    const int order = (int)fm2.at(0);
+    const bool fir = (int)fm2.at(1) == 1;
+    std::vector<double> b, a;
+    if (fir) {
+      b.assign(fm2.begin() + 2, fm2.begin() + 2 + order);
+    } else {
+      a.assign(fm2.begin() + 2, fm2.begin() + 2 + order);
+      b.assign(fm2.begin() + 2 + order, fm2.begin() + 2 + 2 * order);
+    }
+    const int ndc = (int)std::lround(Fs * 2e-3f);
+    const double dcv = (double)(1.0f / (Fs * 2e-3f));
+    std::vector<double> dc(ndc, dcv);
+    std::vector<double> ihcv = par.has("IHC_Vector") ? par.vec<double>("IHC_Vector") : std::vector<double>{8.0};
+    const std::vector<float> ihc = expand(ihcv, kSections);
+    std::vector<double> dmg(kSections);
+    for (int i = 0; i < kSections; ++i) dmg[i] = std::pow(10.0, (double)ihc[i]);
+    const float nc[9] = {60.f, 3.f, 0.1f, 0.f, 0.f, 0.f, 0.61f, 0.23f, 0.16f};   // spont, -, W
+    const float sat = par.scalar<float>("Lambda_SAT", 500.f);
+    ok(IHCNewKernel(dmg.data(), nc, dc.data(), ndc, b.data(), fir ? nullptr : a.data(), fir ? 0 : 1, (int)b.size(),
+                    kSections, TB, par.scalar<double>("SPLRef", 2e-5), nullptr, 0, total, total, total, block, 0, 0,
+                    sat, 1.f, 100.f, 1, 1, 0, 0, 0, 0, 1.f, 1, 1, 0, DF, 0),
+       "IHCNewKernel");
+    ok(enableloadAihc(), "enableloadAihc");
+    std::vector<float> aihc(kLambdas * kSections);
+    const float A3[3] = {70.f, 50.f, 30.f};
+    for (int j = 0; j < kLambdas; ++j)
+      for (int i = 0; i < kSections; ++i) aihc[j * kSections + i] = A3[j];
+    ok(loadAihc(aihc.data()), "loadAihc");
+    const auto t0 = std::chrono::steady_clock::now();
+    ok(RunIHCKernel(nullptr, 0, 1, 0, 0), "RunIHCKernel");
+    ok(cm_device_synchronize(), "cm_device_synchronize");
+    ihc_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    std::vector<float> lam((size_t)kLambdas * total * kSections);
+    ok(BMOHCKernel_Copy_Lambda(lam.data(), lam.size(), 0), "BMOHCKernel_Copy_Lambda");
+    for (int j = 0; j < kLambdas; ++j)
+      if (allowed & (2u << j))
+        out.write(lam.data() + ((size_t)j * total + overlap) * kSections, rows * kSections);
+  }
+  // timers_marked: this host's stage times in ms (BM kernel, IHC/lambda)
+  const float timers[2] = {bm_ms, ihc_ms};
+  out.write(timers, 2);
+  std::printf("cochlea_run: %d time blocks, %zu rows written to %s (BM %.2f ms, IHC/lambda %.2f ms)\n", TB, rows,
+              out_path.c_str(), bm_ms, ihc_ms);
+  ok(BMOHCKernel_Free(), "BMOHCKernel_Free");
+  ok(IHCKernel_Free(), "IHCKernel_Free");
+  return 0;
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+  try {
+    if (argc == 3 && std::string(argv[1]) == "--parse") {
+      ParFile par(argv[2]);
+      for (const auto &kv : par.map()) std::printf("%s = %s\n", kv.first.c_str(), kv.second.c_str());
+      return 0;
+    }
+    if (argc != 2) {
+      std::fprintf(stderr, "usage: %s <file.par> | --parse <file.par>\n", argv[0]);
+      return 2;
+    }
+    ParFile par(argv[1]);
+    return run(par, dir_of(argv[0]));
+  } catch (const std::exception &e) {
+    std::fprintf(stderr, "cochlea_run: %s\n", e.what());
+    return 1;
+  }
+}

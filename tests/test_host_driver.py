@@ -1,0 +1,92 @@
+"""cochlea_run, the C++ command-line host: the parameter-file reader on the
+CPU (ConfigFileHandler semantics) and, on the GPU, the .bin stream of a VOICE
+run against the Python mirror of Run_Cuda and the CPU oracle."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "cochlearmodel_amd", "cochlea_run")
+FS = 44100.0
+
+
+def _parse(tmp_path, text):
+    p = tmp_path / "t.par"
+    p.write_bytes(text.encode())
+    r = subprocess.run([EXE, "--parse", str(p)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    out = {}
+    for line in r.stdout.splitlines():
+        k, v = line.split(" = ", 1)
+        out[k] = v
+    return out
+
+
+def test_parameter_file_semantics(tmp_path):
+    """src/ConfigFileHandler.cpp:32-56: key = value or key = "value" per line,
+    first occurrence wins, CRLF accepted, empty or stray-quote values ignored."""
+    assert os.path.exists(EXE), "cochlea_run is not built (python -m cochlearmodel_amd.build)"
+    kv = _parse(tmp_path, 'A = 1\r\nB=  2 3  4\r\nA = 5\r\nName = "quoted value"\r\n'
+                          'no equals here\r\nC_1=\r\nD =x"y\r\nVec = 1.5 abc 7\n  Indented_Key =\t9\n')
+    assert kv == {"A": "1", "B": "2 3  4", "Name": "quoted value", "Vec": "1.5 abc 7", "Indented_Key": "9"}
+
+
+def test_run_without_input_fails_cleanly(tmp_path):
+    p = tmp_path / "bad.par"
+    p.write_text("Output_Result = x.bin\n")
+    r = subprocess.run([EXE, str(p)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "no input" in r.stderr
+
+
+@pytest.mark.gpu
+def test_voice_run_bin_stream(tmp_path, oracle):
+    """Stream order and offsets of the .bin (failed nodes/blocks, convergence
+    data, BM velocity, three lambdas, timers) against the same run through the
+    Python mirror, and the velocity segment against the oracle, bit for bit."""
+    from cochlearmodel_amd import _lib, solver, stages, workloads
+    print("case: C1 tone through cochlea_run, Convergence_Tests=15, Allowed_Outputs=15", flush=True)
+    sig = workloads.c1_signal()
+    inp = tmp_path / "c1.f64"
+    sig.astype(np.float64).tofile(inp)
+    out = tmp_path / "c1.bin"
+    (tmp_path / "c1.par").write_text(
+        f"Input_File = {inp}\nFs = 44100\nTime_Blocks = 5\nTime_Block_Length = 0.02\nOverlap_Time = 0.005\n"
+        f"OHC_Vector = 0.5\nIHC_Vector = 8\nAllowed_Outputs = 15\nConvergence_Tests = 15\n"
+        f'Output_Result = "{out}"\n')
+    r = subprocess.run([EXE, str(tmp_path / "c1.par")], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = np.fromfile(out, np.float32)
+    # the same run through the Python mirror, then the library's buffers
+    lay = solver.voice_layout(FS, 5, 0.02, 0.005)
+    g = solver.run_bm_voice(sig, lay)
+    L = _lib.lib()
+    bufP1, TB, ov, tot = lay.bufP1, lay.time_blocks, lay.overlap_nodes, lay.total_nodes
+
+    def dev(getter, n, dtype):
+        a = np.zeros(n, dtype)
+        _lib.check(L.GeneralKernel_Copy_Results_Bytes(_lib.ptr(a), getter(), a.nbytes, 0))
+        return a
+
+    ct = np.zeros(bufP1, np.float32)
+    _lib.check(L.extractConvergenceTimes(_lib.ptr(ct), bufP1))
+    parts = [ct[ov:tot], dev(L.getCudaConvergeSpeedBlocks, TB, np.float32),
+             dev(L.getCudaFailedTimeNodes, bufP1, np.int32)[ov:tot].astype(np.float32),
+             dev(L.getCudaFailedBlocks, TB, np.int32).astype(np.float32),
+             dev(L.getCudaConvergedTimeNodes, bufP1, np.float32)[ov:tot],
+             dev(L.getCudaConvergedBlocks, TB, np.float32),
+             dev(L.getCudaConvergedJacobyLoopsPerIteration, bufP1, np.float32)[ov:tot],
+             dev(L.getCudaConvergedJacobyLoopsPerIterationBlocks, TB, np.float32),
+             g["speeds"][ov:tot].ravel()]
+    lam, _ = stages.run_ihc_reference(tot, lay.block_nodes, TB, 0)
+    parts += [lam[j, ov:tot].ravel() for j in range(3)]
+    expect = np.concatenate(parts)
+    assert got.size == expect.size + 2          # + timers (BM, IHC/lambda ms)
+    np.testing.assert_array_equal(got[:-2], expect)
+    # velocity segment against the oracle
+    st, mo, me, ws = oracle.block_decomposition(TB, bufP1, ov, 0)
+    o = oracle.bm_run(oracle.default_scalars(FS), oracle.section_params(), np.full(256, 0.5, np.float32),
+                      oracle.make_items(st, mo, me, ws, 1e-10, 1e-15), g["input"], bufP1, "f32", threads=8)
+    off = sum(p.size for p in parts[:8])
+    np.testing.assert_array_equal(got[off:off + (tot - ov) * 256], o["speeds"][ov:tot].ravel())
