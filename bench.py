@@ -58,6 +58,14 @@ def variant_scalars(name: str):
 
 VARIANTS = ("reference_20_2", "plain_tol", "extrapolated_tol", "learned_tol")
 MLP_FLOP_PER_STEP = 2.0 * (769 * 512 + 512 * 512 + 512 * 256)   # forward, one block-step
+# HBM bytes per launch on the 256-tone batch from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE (KiB per dispatch), scaled by what the counters read for a 1 GiB
+# device copy (FETCH x 2.0, WRITE x 1.0); profiles/pmc_traffic_c2.md
+MEASURED_HBM_BYTES = {
+    "learned_tol": (32524.25 * 2.0 + 16498707.875) * 1024,
+    "reference_20_2": (5948.9 * 2.0 + 2258260.0) * 1024,
+    "ihc_lambda": (1151030.6 * 2.0 + 4515840.0 + 2289617.2 * 2.0 + 6773760.0) * 1024,
+}
 PEAK_BF16_TFLOPS = 2500.0    # MI355X dense BF16 MFMA
 
 
@@ -303,11 +311,14 @@ def main():
         "bm_ms": bm_avg, "ihc_lambda_ms": float(np.mean(ihc_ms)), "input_ms": float(np.mean(in_ms)),
         "bm_section_steps_per_s": sec_steps / (bm_avg * 1e-3),
         "roofline": {"bound": "mfma", "engine": "FP32 VALU", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
+                     "traffic": MEASURED_HBM_BYTES.get(args.variant) if args.signals == 256 else None,
                      "kernel": "bm_kernel", "flop_per_launch": flops},
         "ihc_lambda_roofline": {"bound": "hbm", "achieved": ihc_bytes / (np.mean(ihc_ms) * 1e-3) / 1e9,
                                 "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                                "frac": ihc_bytes / (np.mean(ihc_ms) * 1e-3) / 1e9 / PEAK_HBM_GBS},
+                                "frac": ihc_bytes / (np.mean(ihc_ms) * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                                "traffic": MEASURED_HBM_BYTES["ihc_lambda"] if args.signals == 256 else None,
+                                "fp64_flop_per_launch": work.rows * SECTIONS * 2.0 * (76 + 88)},
     }
     if args.variant == "learned_tol":
         out["roofline"]["kernel"] = "bm_ws_kernel"

@@ -92,6 +92,21 @@ __device__ __forceinline__ void fir_block_steady(const float *__restrict__ src, 
   }
 }
 
+// XCD-aware tile order.  Workgroups are dealt round-robin to the 8 XCDs of
+// the MI355X, each with its own L2; consecutive row tiles share the FIR's tap
+// window, so the linear workgroup id is remapped for every XCD to walk one
+// contiguous range of tiles and find the window rows in its own L2.  Returns
+// (tile x, interval y); a bijection on [0, gx * gy).
+__device__ __forceinline__ void xcd_tile(int &tx, int &ty) {
+  constexpr unsigned kXcd = 8;
+  const unsigned gx = gridDim.x, total = gx * gridDim.y;
+  const unsigned lid = blockIdx.x + blockIdx.y * gx;
+  const unsigned x = lid % kXcd, k = lid / kXcd, per = total / kXcd, rem = total % kXcd;
+  const unsigned logical = x < rem ? x * (per + 1) + k : rem * (per + 1) + (x - rem) * per + k;
+  tx = (int)(logical % gx);
+  ty = (int)(logical / gx);
+}
+
 // Window of kRows outputs starting at interval row r0; returns 1 when every
 // output of the block uses all taps.
 __device__ __forceinline__ int block_windows(const cm_ihc_interval &iv, int64_t r0, int n, int taps, int (&w)[kRows]) {
@@ -109,9 +124,11 @@ __global__ __launch_bounds__(256) void ac_ds_kernel(const float *__restrict__ bm
                                                     const double *__restrict__ b, const double *__restrict__ bpad,
                                                     int nb, float coefA, double coefB, int ac_ready,
                                                     float *__restrict__ AC, float *__restrict__ dS) {
-  const cm_ihc_interval iv = ivs[blockIdx.y];
+  int tx, ty;
+  xcd_tile(tx, ty);
+  const cm_ihc_interval iv = ivs[ty];
   const int s = threadIdx.x;
-  const int64_t r0 = (int64_t)blockIdx.x * kRows;
+  const int64_t r0 = (int64_t)tx * kRows;
   if (r0 >= iv.len) return;
   const int n = iv.len - r0 < kRows ? (int)(iv.len - r0) : kRows;
   float ac[kRows];
@@ -179,9 +196,11 @@ struct LambdaArgs {
 
 // DC box-car FIR, PRE_IHC, IHC and the three lambdas.
 __global__ __launch_bounds__(256) void dc_lambda_kernel(LambdaArgs a) {
-  const cm_ihc_interval iv = a.ivs[blockIdx.y];
+  int tx, ty;
+  xcd_tile(tx, ty);
+  const cm_ihc_interval iv = a.ivs[ty];
   const int s = threadIdx.x;
-  const int64_t r0 = (int64_t)blockIdx.x * kRows;
+  const int64_t r0 = (int64_t)tx * kRows;
   if (r0 >= iv.len) return;
   const int n = iv.len - r0 < kRows ? (int)(iv.len - r0) : kRows;
   const double dmg = a.damage[(int64_t)iv.set * CM_SECTIONS + s];
