@@ -55,6 +55,7 @@ _SIGS = {
     "cm_last_kernel_ms": (C.c_float, []),
     "cm_set_bm_launch": (C.c_int, [C.c_int, C.c_int]),
     "cm_set_bm_order": (C.c_int, [vp, C.c_int]),
+    "cm_set_bm_solver": (C.c_int, [C.c_int, f32, C.c_int, C.c_int, C.c_int]),
     "cm_bm_autotune": (C.c_int, [vp, vp, i64, vp, vp, vp, C.c_int, vp, i64, C.c_int, vp]),
     "BMOHCKernel_Init": (C.c_int, [vp] * 11 + [f32, f32, sz, sz, sz, C.c_int, C.c_int, C.c_int]),
     "BMOHCNewKernel": (C.c_int, [vp, C.c_int, f32, f32, f32, f32, f32, C.c_int, C.c_int, C.c_int, f32, f32, f32,
@@ -129,6 +130,7 @@ _SIGS = {
     "cm_adam": (C.c_int, [vp, vp, vp, vp, vp, i64, f32, f32, f32, f32, C.c_int, f32]),
     "cm_ws_load": (C.c_int, [vp, vp, vp, vp, vp, vp, f32]),
     "cm_ws_set_capture": (C.c_int, [vp, i64, C.c_int, C.c_int]),
+    "cm_ws_load_file": (C.c_int, [C.c_char_p]),
     "cm_ws_targets": (C.c_int, [vp, vp, i64, f32, vp]),
     "cm_ws_release": (C.c_int, []),
 }

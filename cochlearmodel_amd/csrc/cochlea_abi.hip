@@ -71,6 +71,9 @@ struct BMHolder {
   float last_kernel_ms = 0.f;
   int waves_per_block = 1;
   int persistent = 0;
+  // solver options for BMOHCNewKernel launches (cm_set_bm_solver); defaults = reference method
+  int jacobi_mode = 0, check_every = 4, max_sweeps = 1000, warm_start = 0;
+  float tol_rel = 1e-4f;
 };
 BMHolder g_bm;
 
@@ -322,6 +325,21 @@ int cm_bm_autotune(const cm_bm_scalars *sc, const float *d_input, int64_t input_
   return 0;
 }
 
+int cm_set_bm_solver(int jacobi_mode, float tol_rel, int check_every, int max_sweeps, int warm_start) {
+  if (jacobi_mode < 0 || jacobi_mode > 1 || check_every < 1 || max_sweeps < 1 || warm_start < 0 ||
+      warm_start > 2 || (jacobi_mode == 1 && !(tol_rel > 0.f)) || (warm_start != 0 && jacobi_mode != 1)) {
+    set_error("cm_set_bm_solver: jacobi_mode 0|1, tol_rel > 0, check_every/max_sweeps >= 1, warm_start 0..2 "
+              "(warm starts need jacobi_mode 1)");
+    return -3;
+  }
+  g_bm.jacobi_mode = jacobi_mode;
+  g_bm.tol_rel = tol_rel;
+  g_bm.check_every = check_every;
+  g_bm.max_sweeps = max_sweeps;
+  g_bm.warm_start = warm_start;
+  return 0;
+}
+
 int cm_set_bm_launch(int waves_per_block, int persistent) {
   if (waves_per_block != 1 && waves_per_block != 2 && waves_per_block != 4) {
     set_error("waves_per_block must be 1, 2 or 4 (got %d)", waves_per_block);
@@ -439,6 +457,11 @@ int BMOHCNewKernel(const float *input_samples, int override_input_samples, float
   sc.j_slow = JACOBBY_Loops_Slow;
   sc.max_passes = Cuda_Outern_Loops;
   sc.decision_mode = 0;
+  sc.jacobi_mode = g_bm.jacobi_mode;
+  sc.check_every = g_bm.check_every;
+  sc.max_sweeps = g_bm.max_sweeps;
+  sc.tol_rel = g_bm.tol_rel;
+  sc.warm_start = g_bm.warm_start;
 
   std::vector<cm_bm_item> &items = g_bm.host_items;
   block_decomposition(Time_Blocks, samplesBufferLengthP1, overlap_nodes_for_block, Decouple_Filter, items);

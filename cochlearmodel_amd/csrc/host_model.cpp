@@ -88,7 +88,9 @@ int cm_pack_section_params(float *out) {
     out[CM_SEC_SOHC * NS + i] = (float)v[5 * NS + i];
     out[CM_SEC_STM * NS + i] = (float)v[4 * NS + i];
     out[CM_SEC_RTM * NS + i] = (float)v[3 * NS + i];
-    out[CM_SEC_M * NS + i] = (float)m[i];
+    // the Jacobi residual's diagonal, defined from the 1/m row as the
+    // reference's BMOHCKernel_Init interface hands it over (float 1/m)
+    out[CM_SEC_M * NS + i] = (float)(1.0 / (double)out[CM_SEC_RSM * NS + i]);
   }
   return 0;
 }

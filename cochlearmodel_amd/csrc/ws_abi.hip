@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <vector>
 
@@ -149,6 +150,34 @@ int cm_ws_load(const float *W1, const float *b1, const float *W2, const float *b
   g_ws.c_out = c_out;
   g_ws.loaded = true;
   return 0;
+}
+
+int cm_ws_load_file(const char *path) {
+  // float32: W1 (769 x 512), b1, W2 (512 x 512), b2, W3 (512 x 256), b3, c_out
+  const size_t n[6] = {(size_t)kIn[0] * kOut[0], (size_t)kOut[0], (size_t)kIn[1] * kOut[1], (size_t)kOut[1],
+                       (size_t)kIn[2] * kOut[2], (size_t)kOut[2]};
+  size_t total = 1;
+  for (size_t x : n) total += x;
+  std::vector<float> buf(total);
+  FILE *f = path ? std::fopen(path, "rb") : nullptr;
+  if (!f) {
+    set_error("cm_ws_load_file: cannot open %s", path ? path : "(null)");
+    return -1;
+  }
+  const size_t got = std::fread(buf.data(), sizeof(float), total, f);
+  const bool extra = std::fgetc(f) != EOF;
+  std::fclose(f);
+  if (got != total || extra) {
+    set_error("cm_ws_load_file: %s holds %zu floats%s, expected %zu", path, got, extra ? " or more" : "", total);
+    return -3;
+  }
+  const float *p = buf.data();
+  const float *parts[6];
+  for (int i = 0; i < 6; ++i) {
+    parts[i] = p;
+    p += n[i];
+  }
+  return cm_ws_load(parts[0], parts[1], parts[2], parts[3], parts[4], parts[5], *p);
 }
 
 int cm_ws_set_capture(float *d_records, int64_t n_records, int stride, int per_item) {

@@ -28,6 +28,11 @@
 // MAX_TOLERANCE, RELATIVE_ERRORS, M1_SP_Fix_Factor, Tolerance_Fix_Factor, SPLRef,
 // Decouple_Filter, Convergence_Tests, Lambda_SAT, Disable_Lambda,
 // AC_Filter_Vector (Filter_Mode 2 vector: [order, isFIR, (a...), b...]).
+// Keys of this library's solver options (no reference counterpart):
+// Jacobi_Tolerance (> 0: Jacobi to that relative residual instead of the fixed
+// sweeps), Jacobi_Check_Every (4), Jacobi_Max_Sweeps (1000), Warm_Start (0
+// previous pressure, 1 extrapolated, 2 learned) and Warm_Start_Model (default
+// data/warmstart_mlp.bin beside this executable).
 #include <algorithm>
 #include <cctype>
 #include <chrono>
@@ -243,6 +248,13 @@ int run(const ParFile &par, const std::string &exe_dir) {
   const int j_fast = par.scalar<int>("JACOBBY_Loops_Fast", 2), j_slow = par.scalar<int>("JACOBBY_Loops_Slow", 20);
   const int passes = par.scalar<int>("Cuda_Outern_Loops", 10);
   const int DF = par.scalar<int>("Decouple_Filter", 0);
+  const float tau = par.scalar<float>("Jacobi_Tolerance", 0.f);
+  const int warm = par.scalar<int>("Warm_Start", 0);
+  if (warm == 2)
+    ok(cm_ws_load_file(par.str("Warm_Start_Model", exe_dir + "/data/warmstart_mlp.bin").c_str()), "cm_ws_load_file");
+  ok(cm_set_bm_solver(tau > 0.f ? 1 : 0, tau > 0.f ? tau : 1e-4f, par.scalar<int>("Jacobi_Check_Every", 4),
+                      par.scalar<int>("Jacobi_Max_Sweeps", 1000), warm),
+     "cm_set_bm_solver");
   ok(BMOHCNewKernel(input.data(), 0, sc.w_ohc, sc.time_step, sc.time_step_out, sc.delta_x, sc.alpha_r, 1, 1, 0, sc.Ts,
                     sc.alpha_l, sc.alpha_s, sc.Gme, sc.a0, sc.a1, sc.a2, sc.sigma_ow, sc.eta1, sc.eta2, TB, bufP1,
                     overlap, std::lround(overlap / (double)Fs * 1e6), 1, sc.dt_max, sc.dt_min, DF,

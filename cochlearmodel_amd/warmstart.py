@@ -3,8 +3,11 @@ records from the solver itself, exact targets, training on the GPU with the
 library's BF16-MFMA kernels, upload into the solver and evaluation against the
 plain and extrapolated guesses.  No reference counterpart.
 
-    python -m cochlearmodel_amd.warmstart train --out cochlearmodel_amd/data/warmstart_mlp.npz
-    python -m cochlearmodel_amd.warmstart eval --model cochlearmodel_amd/data/warmstart_mlp.npz
+    python -m cochlearmodel_amd.warmstart train --out cochlearmodel_amd/data/warmstart_mlp.bin
+    python -m cochlearmodel_amd.warmstart eval --model cochlearmodel_amd/data/warmstart_mlp.bin
+
+Model file: float32 W1 (769 x 512), b1, W2 (512 x 512), b2, W3 (512 x 256), b3,
+c_out, read by cm_ws_load_file as well; training metadata beside it (.json).
 
 Network input per predictor pass (769 values, each block RMS-normalised):
 predictor displacement d, predictor velocity v, the residual r_lin of the
@@ -30,7 +33,7 @@ from .batch import ToneBatch, random_tones
 from .mlp import SIZES, WarmStartMLP
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-MODEL_PATH = os.path.join(HERE, "data", "warmstart_mlp.npz")
+MODEL_PATH = os.path.join(HERE, "data", "warmstart_mlp.bin")
 FS = 44100.0
 TRAIN_SEED = 20261202 + 51     # training tones
 EVAL_SEED = 20261202 + 52      # held-out tones
@@ -75,14 +78,25 @@ def upload(params, c_out: float):
 
 def save_model(path, params, c_out, meta):
     os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
-    np.savez_compressed(path, W1=params[0], b1=params[1], W2=params[2], b2=params[3], W3=params[4], b3=params[5],
-                        c_out=np.float32(c_out), meta=json.dumps(meta))
+    flat = [np.ascontiguousarray(p, np.float32).ravel() for p in params] + [np.array([c_out], np.float32)]
+    np.concatenate(flat).tofile(path)
+    with open(os.path.splitext(path)[0] + ".json", "w") as f:
+        json.dump(meta, f, indent=1)
 
 
 def load_model(path=MODEL_PATH):
-    z = np.load(path, allow_pickle=False)
-    params = [z[k].astype(np.float32) for k in ("W1", "b1", "W2", "b2", "W3", "b3")]
-    return params, float(z["c_out"]), json.loads(str(z["meta"]))
+    flat = np.fromfile(path, np.float32)
+    params, off = [], 0
+    for kin, kout in zip(SIZES[:-1], SIZES[1:]):
+        params.append(flat[off:off + kin * kout].reshape(kin, kout).copy())
+        off += kin * kout
+        params.append(flat[off:off + kout].copy())
+        off += kout
+    if off + 1 != flat.size:
+        raise ValueError(f"{path}: {flat.size} floats, expected {off + 1}")
+    meta_path = os.path.splitext(path)[0] + ".json"
+    meta = json.load(open(meta_path)) if os.path.exists(meta_path) else {}
+    return params, float(flat[off]), meta
 
 
 def build_training_set(n_signals: int, batch_signals: int, stride: int, per_item: int, tau: float, seed: int,

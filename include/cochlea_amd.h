@@ -51,6 +51,11 @@ int cm_device_synchronize(void);
 float cm_last_kernel_ms(void);
 /* BM launch shape: waves (time blocks) per workgroup, persistent queue. */
 int cm_set_bm_launch(int waves_per_block, int persistent);
+/* Solver of the BMOHCNewKernel launches (no reference counterpart): jacobi_mode
+ * 0 = the reference's fixed sweeps (default), 1 = Jacobi to relative residual
+ * tol_rel checked every check_every sweeps, capped at max_sweeps; warm_start 0
+ * previous pressure, 1 extrapolated, 2 learned (needs cm_ws_load*). */
+int cm_set_bm_solver(int jacobi_mode, float tol_rel, int check_every, int max_sweeps, int warm_start);
 /* Launch order of the next BM launches over an n-block table: wave w
  * integrates block h_order[w] (a permutation; results do not depend on it).
  * NULL clears it. */
@@ -262,6 +267,8 @@ int cm_ws_load(const float *W1, const float *b1, const float *W2, const float *b
  * and, with warm_start 2, the network output (256 at 1288); 1544 floats per
  * record.  d_records = NULL switches capture off. */
 int cm_ws_set_capture(float *d_records, int64_t n_records, int stride, int per_item);
+/* cm_ws_load from a file of float32: W1, b1, W2, b2, W3, b3 (as above) and c_out */
+int cm_ws_load_file(const char *path);
 /* Exact training targets: (A^-1 Y - p_lin) / (rms(r_lin) c_out) per record, A
  * from the m row of d_sec, FP64 inverse.  d_targets: n x 256 floats. */
 int cm_ws_targets(const float *d_sec, const float *d_records, int64_t n, float c_out, float *d_targets);

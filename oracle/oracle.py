@@ -94,7 +94,7 @@ def section_params() -> np.ndarray:
     out[5] = v[5].astype(np.float32)
     out[6] = v[4].astype(np.float32)
     out[7] = v[3].astype(np.float32)
-    out[8] = m.astype(np.float32)
+    out[8] = (1.0 / out[1].astype(np.float64)).astype(np.float32)   # m_i from the float 1/m_i row
     return out
 
 

@@ -90,3 +90,45 @@ def test_voice_run_bin_stream(tmp_path, oracle):
                       oracle.make_items(st, mo, me, ws, 1e-10, 1e-15), g["input"], bufP1, "f32", threads=8)
     off = sum(p.size for p in parts[:8])
     np.testing.assert_array_equal(got[off:off + (tot - ov) * 256], o["speeds"][ov:tot].ravel())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("warm", [1, 2])
+def test_voice_run_tolerance_mode(tmp_path, oracle, warm):
+    """Jacobi_Tolerance / Warm_Start keys: the extrapolated guess is bit-exact
+    against the oracle's same solver; the learned guess (committed network)
+    stays within the tolerance-level distance of the exact per-pass solve."""
+    from cochlearmodel_amd import solver, workloads
+    print(f"case: C1 tone through cochlea_run, tolerance 1e-6, Warm_Start={warm}", flush=True)
+    sig = workloads.c1_signal()
+    inp = tmp_path / "c1.f64"
+    sig.astype(np.float64).tofile(inp)
+    out = tmp_path / "c1.bin"
+    (tmp_path / "c1.par").write_text(
+        f"Input_File = {inp}\nTime_Blocks = 5\nTime_Block_Length = 0.02\nOverlap_Time = 0.005\n"
+        f"Allowed_Outputs = 1\nJacobi_Tolerance = 1e-6\nWarm_Start = {warm}\nOutput_Result = {out}\n")
+    r = subprocess.run([EXE, str(tmp_path / "c1.par")], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = np.fromfile(out, np.float32)
+    lay = solver.voice_layout(FS, 5, 0.02, 0.005)
+    TB, ov, tot, bufP1 = lay.time_blocks, lay.overlap_nodes, lay.total_nodes, lay.bufP1
+    v = got[(tot - ov) + TB:(tot - ov) + TB + (tot - ov) * 256].reshape(tot - ov, 256)
+    x = np.zeros(bufP1, np.float32)
+    x[:len(sig)] = sig
+    st, mo, me, ws = oracle.block_decomposition(TB, bufP1, ov, 0)
+    items = oracle.make_items(st, mo, me, ws, 1e-10, 1e-15)
+    so = oracle.default_scalars(FS)
+    so.jacobi_mode, so.tol_rel, so.check_every, so.max_sweeps, so.warm_start = (1, 1e-6, 4, 1000, 1) if warm == 1 \
+        else (2, 1e-6, 4, 1000, 0)
+    o = oracle.bm_run(so, oracle.section_params(), np.full(256, 0.5, np.float32), items, x, bufP1, "f32", threads=8)
+    ref = o["speeds"][ov:tot]
+    o0 = oracle.bm_run(oracle.default_scalars(FS), oracle.section_params(), np.full(256, 0.5, np.float32), items, x,
+                       bufP1, "f32", threads=8)["speeds"][ov:tot]
+    print(f"driver vs tolerance-mode oracle {np.abs(v - ref).max():.3e}, vs reference-mode oracle "
+          f"{np.abs(v - o0).max():.3e}; {r.stdout.strip()}", flush=True)
+    if warm == 1:
+        np.testing.assert_array_equal(v, ref)
+    else:
+        rel = np.linalg.norm(v - ref) / np.linalg.norm(ref)
+        print(f"learned warm start vs exact solve: rel L2 {rel:.2e}", flush=True)
+        assert rel <= 2e-3
