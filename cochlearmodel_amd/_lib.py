@@ -102,6 +102,8 @@ _SIGS = {
                                    C.c_int, C.c_int, f64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     "cm_jnd_run": (C.c_int, [vp, i64, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, vp, C.c_int, vp, vp, f64, f64,
                              C.c_int, C.c_int, vp, vp, vp]),
+    "cm_complex_jnd_intervals": (C.c_int, [C.c_int, C.c_int, C.c_int, vp]),
+    "cm_complex_jnd": (C.c_int, [vp, vp, C.c_int, vp, C.c_int, C.c_int, C.c_int, f32, vp, vp, vp]),
     "cm_jnd_final": (C.c_int, [vp, vp, C.c_int, C.c_int, C.c_int, vp, C.c_int, vp, f64, f64, C.c_int, C.c_int, vp,
                                vp, vp, vp]),
     "InputProfilesArrayInitialization": (C.c_int, [C.c_int] * 5),

@@ -131,3 +131,87 @@ int cm_default_bm_scalars(float Fs, cm_bm_scalars *o) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- JND ---
+// Aggregated JND of complex (Sim_Type 4) runs: ComplexJNDProfile
+// (src/ComplexJNDProfile.cpp:19-95) driven by AudioGramCreator::calcComplexJND
+// (src/AudioGramCreator.cpp:548-571).  Profile p = noise * n_freq + freq
+// collects the raw JND of its n_levels tested levels in level order.
+extern "C" int cm_complex_jnd_intervals(int n_freq, int n_levels, int n_noises, int32_t *intervals) {
+  // CParams::updateInputProfile (src/params.cpp:693-735): calculated intervals
+  // run noise -> level -> frequency, references are not counted
+  if (n_freq <= 0 || n_levels <= 0 || n_noises <= 0 || !intervals) return -3;
+  for (int nl = 0; nl < n_noises; ++nl)
+    for (int pl = 0; pl < n_levels; ++pl)
+      for (int fl = 0; fl < n_freq; ++fl)
+        intervals[(nl * n_freq + fl) * n_levels + pl] = (nl * n_levels + pl) * n_freq + fl;
+  return 0;
+}
+
+extern "C" int cm_complex_jnd(const double *raw, const int32_t *failed, int n_raw, const int32_t *intervals,
+                              int n_profiles, int n_levels, int methods, float eps, float *jnd_final,
+                              int32_t *warnings, int32_t *conv_warnings) {
+  if (!raw || !intervals || n_profiles <= 0 || n_levels <= 0 || !jnd_final || !warnings || !conv_warnings ||
+      (methods & 3) == 0)
+    return -3;
+  const float kNotFound = 1111.0f;   // MIN_INF_POWER_LEVEL, include/const.h:342
+  int m = 0;
+  for (int bit = 0; bit < 2; ++bit) {
+    if (!(methods & (1 << bit))) continue;
+    for (int p = 0; p < n_profiles; ++p) {
+      const int32_t *iv = intervals + (size_t)p * n_levels;
+      for (int l = 0; l < n_levels; ++l)
+        if (iv[l] < 0 || iv[l] >= n_raw) return -3;
+      auto v = [&](int l) { return raw[iv[l]]; };
+      auto fl = [&](int l) { return failed ? failed[iv[l]] : 0; };
+      float value = kNotFound;
+      int warn = 0, conv = 0;
+      if (bit == 0) {
+        // last strict interior local minimum over the levels
+        int last = -1;
+        for (int l = 1; l + 1 < n_levels; ++l)
+          if (v(l) < v(l - 1) && v(l) < v(l + 1)) last = l;
+        if (last >= 0) {
+          value = (float)v(last);
+          conv = fl(last);
+        } else {
+          warn = 1;
+          // boundary minimum; the reference's convergence flag read here indexes an
+          // empty vector, the chosen interval's own flag is used instead
+          if (n_levels == 1) {
+            value = (float)v(0);
+            conv = fl(0);
+          } else if (v(0) < v(1)) {
+            value = (float)v(0);
+            conv = fl(0);
+          } else if (v(n_levels - 1) < v(n_levels - 2)) {
+            value = (float)v(n_levels - 1);
+            conv = fl(n_levels - 1);
+          }
+        }
+      } else {
+        // first level whose successor is larger by more than eps
+        int at = -1;
+        for (int l = 0; l + 1 < n_levels; ++l)
+          if (v(l + 1) - v(l) > (double)eps) {
+            at = l;
+            break;
+          }
+        if (at >= 0) {
+          value = (float)v(at);
+        } else {
+          value = (float)v(0);
+          warn = 1;
+        }
+        // the reference leaves the profile's convergence flag as the minimum
+        // method set it (0 when that method did not run)
+        conv = (methods & 1) ? conv_warnings[p] : 0;
+      }
+      jnd_final[(size_t)m * n_profiles + p] = value;
+      warnings[(size_t)m * n_profiles + p] = warn;
+      conv_warnings[(size_t)m * n_profiles + p] = conv;
+    }
+    ++m;
+  }
+  return 0;
+}

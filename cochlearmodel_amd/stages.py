@@ -243,3 +243,23 @@ def jnd_final(F_RA, FAI, n_full, n_int, refs, serial, spl_ref=2e-5, max_db=150.0
                              spl_ref, max_db, int(calc_ai), int(calc_rms), ptr(ai), ptr(rate), None, None),
           "cm_jnd_final")
     return ai, rate
+
+
+# ------------------------------------------------------- complex JND ---
+def complex_jnd(raw_db, n_freq, n_levels, n_noises, methods=1, eps=1.0, failed=None):
+    """Aggregated JND per (noise, frequency) profile over the tested levels
+    (AudioGramCreator::calcComplexJND, src/AudioGramCreator.cpp:548-571) through
+    the library's host routine.  Returns (jnd_final, warnings, convergence
+    warnings), each [method][profile]."""
+    raw = np.ascontiguousarray(raw_db, np.float64)
+    n_prof = n_freq * n_noises
+    iv = np.zeros(n_prof * n_levels, np.int32)
+    check(lib().cm_complex_jnd_intervals(n_freq, n_levels, n_noises, ptr(iv)), "cm_complex_jnd_intervals")
+    n_methods = bin(methods & 3).count("1")
+    out = np.zeros(n_methods * n_prof, np.float32)
+    warn = np.zeros(n_methods * n_prof, np.int32)
+    conv = np.zeros(n_methods * n_prof, np.int32)
+    fl = None if failed is None else np.ascontiguousarray(failed, np.int32)
+    check(lib().cm_complex_jnd(ptr(raw), ptr(fl), len(raw), ptr(iv), n_prof, n_levels, methods, C.c_float(eps),
+                               ptr(out), ptr(warn), ptr(conv)), "cm_complex_jnd")
+    return out.reshape(n_methods, n_prof), warn.reshape(n_methods, n_prof), conv.reshape(n_methods, n_prof)

@@ -189,6 +189,20 @@ int cm_jnd_final(const float *F_RA, const float *FisherAISum, int JNDIntervalsFu
                  int handled, const int *refs, int nrefs, const int *serial, double SPLRefVal, double max_db,
                  int calc_ai, int calc_rms, double *AiJND, double *RateJND, float *FisherAI, float *JND_RA);
 
+/* Complex-JND aggregation (Sim_Type 4), ComplexJNDProfile::calculateMinValue /
+ * calculateGradientMinMaxValue (src/ComplexJNDProfile.cpp:19-95) as driven by
+ * AudioGramCreator::calcComplexJND (src/AudioGramCreator.cpp:548-571).
+ * intervals: n_profiles x n_levels indexes into raw (profile = noise * n_freq +
+ * freq, built by cm_complex_jnd_intervals as CParams::updateInputProfile,
+ * src/params.cpp:693-735); failed: per raw entry (may be NULL); methods bit 0 =
+ * minimum, bit 1 = gradient with threshold eps.  Outputs [method][profile]:
+ * jnd_final (1111 when nothing qualifies), warnings, conv_warnings (the
+ * jnd_final, jnd_final_warnings and jnd_final_signal_convergence_warning vectors
+ * of src/solver.cpp:812-822). */
+int cm_complex_jnd_intervals(int n_freq, int n_levels, int n_noises, int32_t *intervals);
+int cm_complex_jnd(const double *raw, const int32_t *failed, int n_raw, const int32_t *intervals, int n_profiles,
+                   int n_levels, int methods, float eps, float *jnd_final, int32_t *warnings, int32_t *conv_warnings);
+
 /* ---- input generation -------------------------------------------------- */
 /* replace InputProfilesArrayInitialization / Termination (src/cochlea.cu:2391-2415) */
 int InputProfilesArrayInitialization(int maxJNDIntervals, int wn_length, int signal_length, int signal_mode,
