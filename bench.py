@@ -229,6 +229,143 @@ def c3_main(args):
     print(json.dumps(out))
 
 
+def c4_sweep():
+    from cochlearmodel_amd.jnd_sweep import JndSweep
+    freqs = np.geomspace(250.0, 8000.0, 16)
+    levels = np.arange(10.0, 101.0, 10.0)
+    noises = (1111.0, 20.0, 40.0, 60.0, 80.0)        # 1111 = MIN_INF_POWER_LEVEL, no noise
+    return JndSweep(freqs, levels, noises, duration=0.3, methods=3)
+
+
+def c4_oracle_interval(sw, j, threads):
+    """CPU oracle of one decoupled interval of the sweep: its blocks re-based onto
+    their own input range (BM), then IHC/lambda over its rows."""
+    from oracle import oracle as O
+    from cochlearmodel_amd import stages
+    blk = sw.interval_blocks(j)
+    a0 = int(sw.items["start"][blk].min())
+    a1 = int(sw.items["mesi"][blk].max()) + 2
+    inp = sw.d_input[a0:a1].cpu().numpy()
+    it = np.zeros(len(blk), O.ITEM_DTYPE)
+    for f in ("start", "mosi", "mesi", "write_start"):
+        it[f] = sw.items[f][blk] - a0
+    for f in ("tol", "m1tol", "set"):
+        it[f] = sw.items[f][blk]
+    o = O.bm_run(O.default_scalars(FS), O.section_params(), np.full(SECTIONS, 0.5, np.float32), it, inp, a1 - a0,
+                 "f32", threads=threads)
+    r0 = j * sw.interval - a0
+    speeds = o["speeds"][r0:r0 + sw.interval]
+    b, _ = stages.load_ac_filter(FS)
+    lam = O.ihc_lambda(speeds, sw.interval, sw.block * sw.TB, b, stages.dc_filter(FS), stages.ihc_damage(8.0),
+                       stages.aihc_table(), np.array(stages.SPONT_DEFAULT, np.float32), threads=threads)
+    return speeds, lam, int(o["steps"].sum())
+
+
+def c4_main(args):
+    """JND sweep in the Sim_Type 4 layout: 16 tones 250-8000 Hz x 10 levels
+    10-100 dB x 5 noise conditions (none, white noise at 20/40/60/80 dB), 300 ms
+    intervals, plus one reference interval per noise condition; one step =
+    input synthesis + BM + IHC/lambda + JND reductions."""
+    import torch
+    from oracle import oracle as O
+    from cochlearmodel_amd import solver, stages
+    sw = c4_sweep()
+    sc = solver.default_scalars(FS)
+    print(f"# JND sweep ready: {sw.n_prof} intervals, {sw.TB} time blocks, {sw.rows} rows", file=sys.stderr,
+          flush=True)
+    for _ in range(args.warmup):
+        sw.step(sc)
+    torch.cuda.synchronize()
+    times = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        times.append(sw.step(sc))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    th0 = time.perf_counter()
+    res = sw.finish()
+    host_ms = 1e3 * (time.perf_counter() - th0)
+    steps, passes, sweeps = sw.stats()
+    ms_per_step = 1e3 * wall / args.steps
+    avg = {k: float(np.mean([t[k] for t in times])) for k in times[0]}
+    flops = flops_per_launch(steps, passes, sweeps)
+    bm_tf = flops / (avg["bm_ms"] * 1e-3) / 1e12
+    lam_bytes = sw.rows * SECTIONS * 16.0
+    jnd_bytes = sw.n_prof * sw.length * SECTIONS * 24.0       # two passes over 3 lambda planes
+    out = {
+        "metric": "section-steps/s (JND sweep)", "value": SECTIONS * steps / (ms_per_step * 1e-3),
+        "unit": "section-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "JND sweep: 16 tones log-spaced 250-8000 Hz x 10 levels 10-100 dB SPL x 5 noise "
+                               "conditions (none; seeded white noise at 20/40/60/80 dB SPL), 300 ms intervals at "
+                               "44.1 kHz, one reference interval per noise condition, Decouple_Filter=15, "
+                               "reference kernel-17 method; input synthesis + BM + IHC/lambda + JND per step",
+                   "intervals": sw.n_prof, "time_blocks": sw.TB, "rows": sw.rows},
+        "stage_ms": avg, "host_final_and_complex_ms": host_ms, "committed_steps": steps,
+        "jacobi_iters_per_step": sweeps / steps, "profiles_per_s": sw.n_prof / (ms_per_step * 1e-3),
+        "roofline": {"bound": "mfma", "engine": "FP32 VALU", "achieved": bm_tf, "peak": PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": bm_tf / PEAK_FP32_TFLOPS, "traffic": None, "kernel": "bm_kernel",
+                     "flop_per_launch": flops},
+        "ihc_lambda_roofline": {"bound": "hbm", "achieved": lam_bytes / (avg["ihc_ms"] * 1e-3) / 1e9,
+                                "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                                "frac": lam_bytes / (avg["ihc_ms"] * 1e-3) / 1e9 / PEAK_HBM_GBS},
+        "jnd_roofline": {"bound": "hbm", "achieved": jnd_bytes / (avg["jnd_ms"] * 1e-3) / 1e9, "peak": PEAK_HBM_GBS,
+                         "unit": "GB/s", "frac": jnd_bytes / (avg["jnd_ms"] * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                         "bytes_per_launch": jnd_bytes},
+        "complex_jnd": {"min_method_db": [float(x) for x in res["complex"][0][0]],
+                        "gradient_method_db": [float(x) for x in res["complex"][0][1]],
+                        "profiles": "noise-major: noise condition x frequency"},
+    }
+    if not args.no_cpu:
+        # oracle chain on sampled calculated intervals and the reference interval of each
+        threads = min(args.cpu_threads, os.cpu_count() or 1)
+        n_calc = len(sw.calc)
+        pick = sw.calc[np.linspace(0, n_calc - 1, args.c4_intervals).round().astype(int)]
+        need = sorted(set(int(j) for j in pick) | set(int(sw.ref_index[j]) for j in pick))
+        t1 = time.perf_counter()
+        cpu, st_total = {}, 0
+        for j in need:
+            cpu[j] = c4_oracle_interval(sw, j, threads)
+            st_total += cpu[j][2]
+        nIHC = np.full(SECTIONS, 30000.0 / SECTIONS)
+        W = np.array(stages.W_DEFAULT, np.float32)
+        db_cpu = {}
+        for j in pick:
+            r = int(sw.ref_index[j])
+            lam2 = np.concatenate([cpu[int(j)][1], cpu[r][1]], axis=1)
+            dA = np.array([sw.profiles["dA"][j], sw.profiles["dA"][r]], np.float32)
+            _, fra, fai = O.jnd(lam2, 2, sw.interval, sw.head, sw.length, dA, np.array([1, 1], np.int32), nIHC, W,
+                                FS, 0.2 * FS)
+            ai, rate = stages.jnd_final(fra.reshape(-1), fai.reshape(-1), 2, 2, np.array([1], np.int32),
+                                        np.array([0, 0], np.int32))
+            db_cpu[int(j)] = (ai[0], rate[0])
+        dt = time.perf_counter() - t1
+        # GPU rows of the same intervals (the last step's outputs)
+        dv, dl, dmax = 0.0, 0.0, 0.0
+        nv = nl = 0.0
+        for j in need:
+            rs = sw.interval_rows(j)
+            g = sw.d_speeds[rs].cpu().numpy()
+            dv += float(np.sum((g.astype(np.float64) - cpu[j][0]) ** 2))
+            nv += float(np.sum(cpu[j][0].astype(np.float64) ** 2))
+            gl = sw.d_lambda[:, rs].cpu().numpy()
+            dl += float(np.sum((gl.astype(np.float64) - cpu[j][1]) ** 2))
+            nl += float(np.sum(cpu[j][1].astype(np.float64) ** 2))
+        for j in pick:
+            s = int(sw.serial[j])
+            dmax = max(dmax, abs(res["ai_db"][s] - db_cpu[int(j)][0]), abs(res["rate_db"][s] - db_cpu[int(j)][1]))
+        out["cpu_baseline"] = {"value": SECTIONS * st_total / dt, "unit": "section-steps/s", "cores": threads,
+                               "kind": "port", "solver": "reference_20_2",
+                               "sample": f"{len(need)} of {sw.n_prof} intervals ({len(pick)} calculated + their "
+                                         f"references), BM + IHC/lambda + JND, {st_total} committed steps, "
+                                         f"{dt:.2f} s wall, FP32 oracle, OpenMP over blocks and sections",
+                               "parity_rel_l2_bm": (dv / max(nv, 1e-300)) ** 0.5,
+                               "parity_rel_l2_lambda": (dl / max(nl, 1e-300)) ** 0.5,
+                               "parity_max_jnd_db": dmax}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -241,9 +378,10 @@ def main():
     ap.add_argument("--accuracy-blocks", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
-    ap.add_argument("--workload", default="c2", choices=("c2", "c3"),
-                    help="c2: 256 uniform tones (headline); c3: 1024 uneven signals")
+    ap.add_argument("--workload", default="c2", choices=("c2", "c3", "c4"),
+                    help="c2: 256 uniform tones (headline); c3: 1024 uneven signals; c4: JND sweep")
     ap.add_argument("--c3-signals", type=int, default=1024)
+    ap.add_argument("--c4-intervals", type=int, default=8, help="calculated intervals checked on the CPU")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -263,6 +401,8 @@ def main():
         del src, dst
     if args.workload == "c3":
         return c3_main(args)
+    if args.workload == "c4":
+        return c4_main(args)
     work = c2_workload(args.signals)
     model_meta = load_warm_start_model()
     sc = variant_scalars(args.variant)
