@@ -469,6 +469,34 @@ int cm_jnd_final(const float *F_RA, const float *FisherAISum, int JNDIntervalsFu
   return 0;
 }
 
+// Failed_Converged_Signals of calcJNDFinal (src/AudioGramCreator.cpp:459-467,
+// 490-493): every calculated interval adds the failed-block flags of its
+// blocks, Decouple_Filter blocks per interval (all blocks when it is 0), into
+// failed_signals[serial[g]].  The caller zeroes failed_signals once; calls for
+// successive chunks accumulate, as the reference's member vector does.  Block
+// indexes past n_blocks count as 0 (the reference reads past its vector there).
+int cm_jnd_failed_signals(const int32_t *failed_blocks, int n_blocks, int JNDIntervals, int handled, const int *refs,
+                          int nrefs, const int *serial, int decouple_filter, int32_t *failed_signals) {
+  if (!failed_blocks || !failed_signals || n_blocks < 0 || JNDIntervals < 0 || (nrefs > 0 && !refs)) {
+    set_error("cm_jnd_failed_signals: bad argument");
+    return -3;
+  }
+  const int64_t coupler = decouple_filter > 0 ? decouple_filter : n_blocks;
+  for (int i = 0; i < JNDIntervals; ++i) {
+    const int g = i + handled;
+    bool is_ref = false;
+    for (int r = 0; r < nrefs; ++r)
+      if (refs[r] == g) is_ref = true;
+    if (is_ref) continue;
+    const int out = serial ? serial[g] : g;
+    for (int64_t p = 0; p < coupler; ++p) {
+      const int64_t b = p + (int64_t)i * coupler;
+      if (b < n_blocks) failed_signals[out] += failed_blocks[b];
+    }
+  }
+  return 0;
+}
+
 // ---------------------------------------------------------- input gen ---
 // src/cochlea.cu:2391-2415.
 int InputProfilesArrayInitialization(int maxJNDIntervals, int wn_length, int signal_length, int signal_mode,

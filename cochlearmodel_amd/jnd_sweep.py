@@ -67,6 +67,7 @@ class JndSweep:
         self.d_items = torch.from_numpy(items.view(np.uint8).copy()).to(dev)
         self.d_speeds = torch.zeros((self.bufP1, SECTIONS), dtype=torch.float32, device=dev)
         self.d_stats = torch.zeros(self.TB * 3, dtype=torch.int64, device=dev)
+        self.d_failed_blocks = torch.zeros(self.TB, dtype=torch.int32, device=dev)
         self.ihc = stages.BatchedIHC(FS)
         self.fir_ivs, _ = stages.reference_intervals(self.rows, self.block, self.TB, self.DF)
         self.d_lambda = torch.empty((3, self.rows, SECTIONS), dtype=torch.float32, device=dev)
@@ -96,8 +97,8 @@ class JndSweep:
         L.cm_set_stream(self.stream)
         lib.check(L.cm_bm_run(C.byref(sc), lib.ptr(self.d_input), self.d_input.numel(), lib.ptr(self.bm.sec),
                               lib.ptr(self.bm.gamma), lib.ptr(self.d_items), self.TB, lib.ptr(self.d_speeds),
-                              self.bufP1, None, None, None, None, None, None, None, None, lib.ptr(self.d_stats), 1, 0,
-                              1 if timed else 0), "bm")
+                              self.bufP1, None, None, None, None, lib.ptr(self.d_failed_blocks), None, None, None,
+                              lib.ptr(self.d_stats), 1, 0, 1 if timed else 0), "bm")
         return L.cm_last_kernel_ms() if timed else None
 
     def run_lambda(self):
@@ -112,15 +113,18 @@ class JndSweep:
                                lib.ptr(self.d_mean_rate), lib.ptr(self.d_F_RA), lib.ptr(self.d_FAI)), "cm_jnd_run")
 
     def finish(self):
-        """Final dB per calculated interval (host FP64, as calcJNDFinal) and the
-        complex-JND aggregation (rate JND, Calculate_From_Mean_Rate)."""
+        """Final dB and failed-signal counts per calculated interval (host, as
+        calcJNDFinal) and the complex-JND aggregation (rate JND,
+        Calculate_From_Mean_Rate)."""
         F_RA = self.d_F_RA.cpu().numpy().reshape(-1)
         FAI = self.d_FAI.cpu().numpy().reshape(-1)
         ai, rate = stages.jnd_final(F_RA, FAI, self.n_prof, self.n_prof, self.refs, self.serial)
         n_calc = len(self.calc)
+        failed = stages.jnd_failed_signals(self.d_failed_blocks.cpu().numpy(), self.n_prof, self.refs, self.serial,
+                                           self.DF, n_calc)
         final = stages.complex_jnd(rate[:n_calc], len(self.freqs), len(self.levels), len(self.noises), self.methods,
-                                   self.eps)
-        return {"ai_db": ai[:n_calc], "rate_db": rate[:n_calc], "complex": final}
+                                   self.eps, failed)
+        return {"ai_db": ai[:n_calc], "rate_db": rate[:n_calc], "failed_signals": failed, "complex": final}
 
     def step(self, sc, timed_bm: bool = True):
         """Input synthesis + BM + IHC/lambda + JND reductions; per-stage ms."""

@@ -245,6 +245,17 @@ def jnd_final(F_RA, FAI, n_full, n_int, refs, serial, spl_ref=2e-5, max_db=150.0
     return ai, rate
 
 
+def jnd_failed_signals(failed_blocks, n_int, refs, serial, decouple_filter, n_out, handled=0):
+    """Failed-signal counts per calculated interval (calcJNDFinal)."""
+    fb = np.ascontiguousarray(failed_blocks, np.int32)
+    out = np.zeros(n_out, np.int32)
+    refs = np.ascontiguousarray(refs, np.int32)
+    serial = np.ascontiguousarray(serial, np.int32)
+    check(lib().cm_jnd_failed_signals(ptr(fb), len(fb), n_int, handled, ptr(refs), len(refs), ptr(serial),
+                                      decouple_filter, ptr(out)), "cm_jnd_failed_signals")
+    return out
+
+
 # ------------------------------------------------------- complex JND ---
 def complex_jnd(raw_db, n_freq, n_levels, n_noises, methods=1, eps=1.0, failed=None):
     """Aggregated JND per (noise, frequency) profile over the tested levels

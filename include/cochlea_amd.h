@@ -188,6 +188,12 @@ int cm_jnd_run(const float *d_lambda, int64_t rows, int n_intervals, int interva
 int cm_jnd_final(const float *F_RA, const float *FisherAISum, int JNDIntervalsFull, int JNDIntervals,
                  int handled, const int *refs, int nrefs, const int *serial, double SPLRefVal, double max_db,
                  int calc_ai, int calc_rms, double *AiJND, double *RateJND, float *FisherAI, float *JND_RA);
+/* replaces the Failed_Converged_Signals sums of calcJNDFinal
+ * (src/AudioGramCreator.cpp:459-467, 490-493): per calculated interval, the
+ * failed-block flags of its Decouple_Filter blocks, added into
+ * failed_signals[serial[g]] */
+int cm_jnd_failed_signals(const int32_t *failed_blocks, int n_blocks, int JNDIntervals, int handled, const int *refs,
+                          int nrefs, const int *serial, int decouple_filter, int32_t *failed_signals);
 
 /* Complex-JND aggregation (Sim_Type 4), ComplexJNDProfile::calculateMinValue /
  * calculateGradientMinMaxValue (src/ComplexJNDProfile.cpp:19-95) as driven by

@@ -55,3 +55,17 @@ def test_gradient_method_cases():
     val2, warn2, _ = _run(profiles, 3, 1, methods=3, eps=1.0)
     assert val2.shape == (2, 3)
     np.testing.assert_array_equal(val2[1], val[0])
+
+
+def test_failed_signals_per_calculated_interval():
+    from cochlearmodel_amd import stages
+    # 4 intervals of 3 blocks; interval 3 is the reference
+    fb = np.array([0, 1, 0, 0, 0, 0, 1, 1, 1, 1, 0, 0], np.int32)
+    refs = np.array([3], np.int32)
+    serial = np.array([0, 1, 2, 0], np.int32)
+    out = stages.jnd_failed_signals(fb, 4, refs, serial, 3, 3)
+    np.testing.assert_array_equal(out, [1, 0, 3])
+    # second chunk of the same run (handled = 4 intervals) accumulates into the same counts
+    out2 = stages.jnd_failed_signals(np.zeros(3, np.int32), 1, np.zeros(0, np.int32), np.zeros(5, np.int32), 3, 3,
+                                     handled=4)
+    np.testing.assert_array_equal(out2, [0, 0, 0])

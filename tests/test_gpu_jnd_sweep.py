@@ -40,8 +40,9 @@ def test_sweep_matches_oracle_chain(oracle, small_sweep):
     st, mo, me, ws = oracle.block_decomposition(sw.TB, sw.bufP1, sw.ov, sw.DF)
     items = oracle.make_items(st, mo, me, ws, 1e-10, 1e-15)
     o = oracle.bm_run(oracle.default_scalars(FS), oracle.section_params(), np.full(256, 0.5, np.float32), items, inp,
-                      sw.bufP1, "f32", threads=8)
+                      sw.bufP1, "f32", threads=8, diagnostics=True)
     g = sw.d_speeds.cpu().numpy()
+    np.testing.assert_array_equal(sw.d_failed_blocks.cpu().numpy(), o["failed_blocks"])
     np.testing.assert_array_equal(g[:sw.rows], o["speeds"][:sw.rows])
     b, _ = stages.load_ac_filter(FS)
     lo = oracle.ihc_lambda(o["speeds"][:sw.rows], sw.interval, sw.block * sw.TB, b, stages.dc_filter(FS),
@@ -64,7 +65,9 @@ def test_sweep_matches_oracle_chain(oracle, small_sweep):
     _say(f"JND dB max diff: rate {d_rate:.2e}, AI {d_ai:.2e}; rate dB {np.round(res['rate_db'], 2)}")
     assert d_rate <= 0.01 and d_ai <= 0.01
     assert np.all(np.isfinite(res["rate_db"]))
-    val_o, warn_o, _ = stages.complex_jnd(rate_o[:n_calc], 2, 3, 2, 3, 1.0)
+    failed_o = stages.jnd_failed_signals(o["failed_blocks"], sw.n_prof, sw.refs, sw.serial, sw.DF, n_calc)
+    np.testing.assert_array_equal(res["failed_signals"], failed_o)
+    val_o, warn_o, _ = stages.complex_jnd(rate_o[:n_calc], 2, 3, 2, 3, 1.0, failed_o)
     val, warn, _ = res["complex"]
     _say(f"complex JND gpu {val.tolist()} oracle {val_o.tolist()}")
     np.testing.assert_allclose(val, val_o, atol=0.01)
