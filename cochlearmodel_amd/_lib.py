@@ -92,6 +92,9 @@ _SIGS = {
     "releaseBuffer": (C.c_int, [C.c_int]),
     "RunIHCKernel": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int]),
     "BMOHCKernel_Copy_Lambda": (C.c_int, [vp, sz, C.c_int]),
+    "ReverseKernel_Copy_Results_Float": (C.c_int, [vp, vp, sz, sz, C.c_int]),
+    "updateCUDALambdaArray": (C.c_int, [vp, vp, sz, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "cm_fix_lambda_run": (C.c_int, [vp, vp, C.c_int64, vp]),
     "getCudaLambda": (vp, []),
     "getCudaBuffer": (vp, []),
     "getCudaMeanRate": (vp, []),

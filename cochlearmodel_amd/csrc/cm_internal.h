@@ -116,6 +116,8 @@ extern "C" int cm_launch_ihc_lambda(const float *d_bm, int64_t rows, const cm_ih
                                     const double *d_ac_a, int n_ac, const double *d_dc, const double *d_dcpad, int n_dc, const double *d_damage, const float *d_A,
                                     const float *spont3, float lambda_sat, float eta_ac, float eta_dc, float scale_bm,
                                     float *d_scratch, float *d_lambda, float *d_stages, hipStream_t stream);
+extern "C" int cm_launch_fix_lambda(float *d_lambda, float *d_buffer, int64_t rows, const float *spont3,
+                                    hipStream_t stream);
 extern "C" int cm_launch_jnd(const cm::JndArgs *a, hipStream_t stream);
 extern "C" int cm_launch_calc_dA(const float *input, float *dA, int n, int interval_nodes, int head_nodes,
                                  int length_nodes, int offset_start, hipStream_t stream);

@@ -284,3 +284,35 @@ extern "C" int cm_launch_ihc_lambda(const float *d_bm, int64_t rows, const cm_ih
   hipLaunchKernelGGL(dc_lambda_kernel, dim3((unsigned)gx, n_ivs), dim3(256), 0, stream, a);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
+
+// Spontaneous-rate fix-up of lambda loaded for a run that resumes from the
+// lambda stage (CUDAFIXJND_Lambda, src/cochlea.cu:4512-4524): per fibre group
+// k, buffer = max(spont_k, lambda) when kept, lambda = max(0, lambda - spont_k).
+// Elementwise over [3][rows][256]; 16-byte accesses, grid-stride.
+namespace cm {
+__global__ __launch_bounds__(256) void fix_lambda_kernel(float4 *__restrict__ lam, float4 *__restrict__ buf,
+                                                        int64_t plane4, float s0, float s1, float s2) {
+  const int64_t n4 = 3 * plane4;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) {
+    const int k = (int)(i / plane4);
+    const float s = k == 0 ? s0 : (k == 1 ? s1 : s2);
+    const float4 v = lam[i];
+    if (buf) buf[i] = make_float4(fmaxf(s, v.x), fmaxf(s, v.y), fmaxf(s, v.z), fmaxf(s, v.w));
+    lam[i] = make_float4(fmaxf(0.0f, v.x - s), fmaxf(0.0f, v.y - s), fmaxf(0.0f, v.z - s), fmaxf(0.0f, v.w - s));
+  }
+}
+}  // namespace cm
+
+// lambda and buffer: [3][rows][256] floats (16-byte aligned device memory).
+extern "C" int cm_launch_fix_lambda(float *d_lambda, float *d_buffer, int64_t rows, const float *spont3,
+                                    hipStream_t stream) {
+  if (rows <= 0) return 0;
+  const int64_t plane4 = rows * (CM_SECTIONS / 4);
+  int64_t blocks = (3 * plane4 + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(cm::fix_lambda_kernel, dim3((unsigned)blocks), dim3(256), 0, stream,
+                     reinterpret_cast<float4 *>(d_lambda), reinterpret_cast<float4 *>(d_buffer), plane4, spont3[0],
+                     spont3[1], spont3[2]);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}

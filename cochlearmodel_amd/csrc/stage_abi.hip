@@ -59,6 +59,9 @@ struct StageHolder {
   float sat = 500.f, eta_ac = 1.f, eta_dc = 100.f, scale = 1.f;
   int calc_time = 0, write_time = 0, interval_nodes = 0, time_blocks = 1, decouple = 0;
   int aihc_loaded = 0;
+  // lambda nodes the copy buffer holds for the current run (0: not filled,
+  // lambda copies then read the lambda planes themselves)
+  size_t buffer_valid = 0;
   double wn_dc = 0, wn_norm = 1, sig_dc = 0, sig_norm = 1;
 };
 StageHolder g_st;
@@ -152,6 +155,7 @@ int allocateBuffer(int size_in_nodes, int Show_Run_Time) {
 int releaseBuffer(int Show_Run_Time) {
   (void)Show_Run_Time;
   g_st.buffer1.release();
+  g_st.buffer_valid = 0;
   return 0;
 }
 
@@ -213,11 +217,13 @@ int RunIHCKernel(float *host_backup, int Show_Run_Time, int save_lambda, int bac
     set_error("RunIHCKernel: launch failed (%d)", rc);
     return -1;
   }
+  g_st.buffer_valid = 0;
   if (save_lambda) {
     if (g_st.buffer1.ensure(3 * plane)) return -1;
     if (check(hipMemcpyAsync(g_st.buffer1.ptr, g_st.lambda.ptr, 3 * plane * sizeof(float), hipMemcpyDeviceToDevice,
                              stream()), "save lambda"))
       return -1;
+    g_st.buffer_valid = 3 * plane;
   }
   if (want_stages) {
     // AC=9, DC=10, dS=11/Shigh=12, IHC=13, PRE_IHC=14 (src/params.cpp:165-187)
@@ -230,14 +236,88 @@ int RunIHCKernel(float *host_backup, int Show_Run_Time, int save_lambda, int bac
   return check(hipStreamSynchronize(stream()), "RunIHCKernel");
 }
 
-// src/cochlea.cu:834-837.
+// src/cochlea.cu:834-837: copies from the saved-lambda buffer.  When this run
+// filled it (RunIHCKernel with save_lambda, or the lambda-stage fix-up with a
+// buffer update) the buffer is read, as in the reference; otherwise the
+// lambda planes themselves.
 int BMOHCKernel_Copy_Lambda(float *target, size_t lambdaNodes, int offset) {
-  if ((size_t)offset + lambdaNodes > g_st.lambda.cap) {
-    set_error("BMOHCKernel_Copy_Lambda: %d+%zu exceeds %zu", offset, lambdaNodes, g_st.lambda.cap);
+  const bool from_buffer = g_st.buffer_valid > 0;
+  const float *src = from_buffer ? g_st.buffer1.ptr : g_st.lambda.ptr;
+  const size_t cap = from_buffer ? g_st.buffer_valid : g_st.lambda.cap;
+  if (offset < 0 || (size_t)offset + lambdaNodes > cap) {
+    set_error("BMOHCKernel_Copy_Lambda: %d+%zu exceeds %zu", offset, lambdaNodes, cap);
     return -3;
   }
-  return check(hipMemcpy(target, g_st.lambda.ptr + offset, lambdaNodes * sizeof(float), hipMemcpyDeviceToHost),
-               "copy lambda");
+  return check(hipMemcpy(target, src + offset, lambdaNodes * sizeof(float), hipMemcpyDeviceToHost), "copy lambda");
+}
+
+// ReverseKernel_Copy_Results_Template<JNDFloat> (src/cochlea.cu:3691-3693):
+// host rows [start_time_node, start_time_node + time_nodes) of cpu_src to the
+// start of cuda_target, which must be one of this library's stage buffers
+// (getCudaLambda, getCudaBuffer) or the BM velocity buffer.
+int ReverseKernel_Copy_Results_Float(const float *cpu_src, float *cuda_target, size_t start_time_node,
+                                     size_t time_nodes, int sections) {
+  if (!cpu_src || !cuda_target || sections <= 0) {
+    set_error("ReverseKernel_Copy_Results_Float: bad argument");
+    return -3;
+  }
+  const size_t n = time_nodes * (size_t)sections;
+  size_t cap = 0;
+  if (cuda_target == g_st.lambda.ptr) cap = g_st.lambda.cap;
+  else if (cuda_target == g_st.buffer1.ptr) cap = g_st.buffer1.cap;
+  else if (cuda_target == bm_speeds_ptr()) cap = bm_result_nodes();
+  else {
+    set_error("ReverseKernel_Copy_Results_Float: target is not a buffer of this library");
+    return -3;
+  }
+  if (n > cap) {
+    set_error("ReverseKernel_Copy_Results_Float: %zu nodes exceed the target's %zu", n, cap);
+    return -3;
+  }
+  return check(hipMemcpy(cuda_target, cpu_src + start_time_node * (size_t)sections, n * sizeof(float),
+                         hipMemcpyHostToDevice), "reverse copy");
+}
+
+// updateCUDALambdaArray<JNDFloat> (src/cochlea.cu:4526-4540) on the lambda
+// planes of calc_time_nodes rows: lambda = max(0, lambda - spont_k) and, when
+// cuda_buffer_update, the saved-lambda buffer = max(spont_k, lambda).
+int updateCUDALambdaArray(float *lambda_array, float *cuda_buffer, size_t calc_time_nodes, int sections,
+                          int Show_Run_Time, int Show_Device_Data, int cuda_buffer_update) {
+  (void)Show_Run_Time; (void)Show_Device_Data;
+  const size_t n = 3 * calc_time_nodes * CM_SECTIONS;
+  if (sections != CM_SECTIONS || lambda_array != g_st.lambda.ptr || n > g_st.lambda.cap) {
+    set_error("updateCUDALambdaArray: needs the lambda planes of this library (%d sections, %zu rows)", CM_SECTIONS,
+              calc_time_nodes);
+    return -3;
+  }
+  float *buf = nullptr;
+  if (cuda_buffer_update) {
+    if (cuda_buffer && cuda_buffer != g_st.buffer1.ptr) {
+      set_error("updateCUDALambdaArray: buffer is not this library's saved-lambda buffer");
+      return -3;
+    }
+    if (g_st.buffer1.ensure(n)) return -1;
+    buf = g_st.buffer1.ptr;
+  }
+  if (cm_launch_fix_lambda(g_st.lambda.ptr, buf, (int64_t)calc_time_nodes, g_st.spont, stream())) {
+    set_error("updateCUDALambdaArray: launch failed");
+    return -1;
+  }
+  g_st.buffer_valid = cuda_buffer_update ? n : 0;
+  return check(hipStreamSynchronize(stream()), "updateCUDALambdaArray");
+}
+
+// Batched lambda fix-up over device pointers: [3][rows][256] (buffer may be NULL).
+int cm_fix_lambda_run(float *d_lambda, float *d_buffer, int64_t rows, const float *spont3) {
+  if (!d_lambda || !spont3 || rows < 0) {
+    set_error("cm_fix_lambda_run: bad argument");
+    return -3;
+  }
+  if (((uintptr_t)d_lambda | (uintptr_t)d_buffer) & 15) {
+    set_error("cm_fix_lambda_run: buffers must be 16-byte aligned");
+    return -3;
+  }
+  return cm_launch_fix_lambda(d_lambda, d_buffer, rows, spont3, stream()) ? -1 : 0;
 }
 
 void *getCudaLambda(void) { return g_st.lambda.ptr; }
@@ -679,6 +759,7 @@ int IHCKernel_Free(void) {
   g_st.lambda.release();
   g_st.scratch.release();
   g_st.buffer1.release();
+  g_st.buffer_valid = 0;
   g_st.stages.release();
   g_st.ivs.release();
   g_st.iir_ivs.release();

@@ -28,6 +28,10 @@
 // MAX_TOLERANCE, RELATIVE_ERRORS, M1_SP_Fix_Factor, Tolerance_Fix_Factor, SPLRef,
 // Decouple_Filter, Convergence_Tests, Lambda_SAT, Disable_Lambda,
 // AC_Filter_Vector (Filter_Mode 2 vector: [order, isFIR, (a...), b...]).
+// Run_Stage_Calculation (1: from BM velocity, [totalTimeNodes][256] floats;
+// 2: from lambda, three such planes) with Run_Stage_File_name (raw float32) or
+// Run_Stage_Vector: the stages before it are skipped, as are the convergence
+// and failure vectors, which only a run from the input produces.
 // Keys of this library's solver options (no reference counterpart):
 // Jacobi_Tolerance (> 0: Jacobi to that relative residual instead of the fixed
 // sweeps), Jacobi_Check_Every (4), Jacobi_Max_Sweeps (1000), Warm_Start (0
@@ -144,6 +148,29 @@ std::vector<double> read_float64_file(const std::string &path) {
   return v;
 }
 
+// CParams::get_stage_data (src/params.cpp:1710-1745): n floats of the stage
+// data from element `offset` on, from Run_Stage_Vector or the raw float32 file
+// Run_Stage_File_name (TBin::read_padd, include/TBin.tpp:138-196: what the file
+// lacks stays 0).
+std::vector<float> stage_data(const ParFile &par, size_t offset, size_t n) {
+  std::vector<float> out(n, 0.f);
+  if (par.has("Run_Stage_Vector")) {
+    const std::vector<float> v = par.vec<float>("Run_Stage_Vector");
+    for (size_t i = 0; i < n && offset + i < v.size(); ++i) out[i] = v[offset + i];
+    return out;
+  }
+  const std::string path = par.str("Run_Stage_File_name");
+  std::ifstream f(path, std::ios::binary);
+  if (!f) throw std::runtime_error("cannot open stage file " + path);
+  f.seekg(0, std::ios::end);
+  const size_t len = (size_t)f.tellg() / sizeof(float);
+  if (offset < len) {
+    f.seekg((std::streamoff)(offset * sizeof(float)));
+    f.read(reinterpret_cast<char *>(out.data()), (std::streamsize)(std::min(n, len - offset) * sizeof(float)));
+  }
+  return out;
+}
+
 // the committed default AC filter: the numbers of "filter_mode_2": [ ... ]
 std::vector<double> read_filter_fixture(const std::string &path) {
   std::ifstream f(path);
@@ -194,9 +221,20 @@ int run(const ParFile &par, const std::string &exe_dir) {
   const long overlap_us = (long)std::llround(overlap_time * 1000.0) * 1000;
   const int overlap = (int)(overlap_us * (double)Fs / 1000000.0);
 
+  // Run_Stage_Calculation (include/params.h:249, 390-395): 0 from the input,
+  // 1 from BM velocity, 2 from lambda, read from Run_Stage_File_name or
+  // Run_Stage_Vector
+  const int run_stage = par.scalar<int>("Run_Stage_Calculation", 0);
+  if (run_stage < 0 || run_stage > 2) throw std::runtime_error("Run_Stage_Calculation must be 0, 1 or 2");
+  if (run_stage > 0 && !par.has("Run_Stage_Vector") && par.str("Run_Stage_File_name").empty())
+    throw std::runtime_error("Chosen Run stage calculation (=" + std::to_string(run_stage) + ") need data or file");
+
   // input signal (Input_Signal, Input_File, or Sin_Frequency/Sin_dB)
   std::vector<double> signal;
-  if (par.has("Input_Signal")) {
+  if (run_stage > 0) {
+    if (!par.has("Time_Blocks") && !par.has("Duration"))
+      throw std::runtime_error("a run from a stage needs Time_Blocks or Duration");
+  } else if (par.has("Input_Signal")) {
     signal = par.vec<double>("Input_Signal");
   } else if (par.has("Input_File") && !par.str("Input_File").empty()) {
     signal = read_float64_file(par.str("Input_File"));
@@ -255,46 +293,54 @@ int run(const ParFile &par, const std::string &exe_dir) {
   ok(cm_set_bm_solver(tau > 0.f ? 1 : 0, tau > 0.f ? tau : 1e-4f, par.scalar<int>("Jacobi_Check_Every", 4),
                       par.scalar<int>("Jacobi_Max_Sweeps", 1000), warm),
      "cm_set_bm_solver");
-  ok(BMOHCNewKernel(input.data(), 0, sc.w_ohc, sc.time_step, sc.time_step_out, sc.delta_x, sc.alpha_r, 1, 1, 0, sc.Ts,
-                    sc.alpha_l, sc.alpha_s, sc.Gme, sc.a0, sc.a1, sc.a2, sc.sigma_ow, sc.eta1, sc.eta2, TB, bufP1,
-                    overlap, std::lround(overlap / (double)Fs * 1e6), 1, sc.dt_max, sc.dt_min, DF,
-                    par.scalar<float>("MAX_M1_SP_ERROR", 1e-15f), par.scalar<float>("MAX_TOLERANCE", 1e-10f),
-                    par.scalar<int>("RELATIVE_ERRORS", 0), 1, par.scalar<float>("M1_SP_Fix_Factor", 0.f),
-                    par.scalar<float>("Tolerance_Fix_Factor", 0.f), par.scalar<float>("SPLRef", 2e-5f), 0, 0, 1,
-                    j_fast, j_slow, passes, 17, 2),
-     "BMOHCNewKernel");
-  const float bm_ms = cm_last_kernel_ms();
-
-  std::vector<float> speeds((size_t)bufP1 * kSections);
-  ok(BMOHCKernel_Copy_Results(speeds.data(), speeds.size(), 0), "BMOHCKernel_Copy_Results");
-  auto dev_copy = [&](void *src, size_t n_bytes, void *dst) {
-    ok(GeneralKernel_Copy_Results_Bytes(dst, src, n_bytes, 0), "GeneralKernel_Copy_Results_Bytes");
-  };
-  std::vector<int32_t> failed_nodes(bufP1), failed_blocks(TB);
-  dev_copy(getCudaFailedTimeNodes(), failed_nodes.size() * 4, failed_nodes.data());
-  dev_copy(getCudaFailedBlocks(), failed_blocks.size() * 4, failed_blocks.data());
-  std::vector<float> conv_times(bufP1), conv_times_b(TB), conv_loops(bufP1), conv_loops_b(TB), conv_j(bufP1),
-      conv_j_b(TB);
-  ok(extractConvergenceTimes(conv_times.data(), conv_times.size()), "extractConvergenceTimes");
-  dev_copy(getCudaConvergeSpeedBlocks(), TB * 4, conv_times_b.data());
-  dev_copy(getCudaConvergedTimeNodes(), bufP1 * 4, conv_loops.data());
-  dev_copy(getCudaConvergedBlocks(), TB * 4, conv_loops_b.data());
-  dev_copy(getCudaConvergedJacobyLoopsPerIteration(), bufP1 * 4, conv_j.data());
-  dev_copy(getCudaConvergedJacobyLoopsPerIterationBlocks(), TB * 4, conv_j_b.data());
-
+  std::vector<float> speeds((size_t)bufP1 * kSections, 0.f);
+  float bm_ms = 0.f;
   BinWriter out(out_path);
   const size_t rows = (size_t)(total - overlap);
-  if (conv_tests & 1) out.write(conv_times.data() + overlap, rows);
-  if (conv_tests & 2) out.write(conv_times_b.data(), TB);
-  out.write_cast(failed_nodes.data() + overlap, rows);
-  out.write_cast(failed_blocks.data(), TB);
-  if (conv_tests & 4) {
-    out.write(conv_loops.data() + overlap, rows);
-    out.write(conv_loops_b.data(), TB);
-  }
-  if (conv_tests & 8) {
-    out.write(conv_j.data() + overlap, rows);
-    out.write(conv_j_b.data(), TB);
+  if (run_stage == 0) {
+    ok(BMOHCNewKernel(input.data(), 0, sc.w_ohc, sc.time_step, sc.time_step_out, sc.delta_x, sc.alpha_r, 1, 1, 0,
+                      sc.Ts, sc.alpha_l, sc.alpha_s, sc.Gme, sc.a0, sc.a1, sc.a2, sc.sigma_ow, sc.eta1, sc.eta2, TB,
+                      bufP1, overlap, std::lround(overlap / (double)Fs * 1e6), 1, sc.dt_max, sc.dt_min, DF,
+                      par.scalar<float>("MAX_M1_SP_ERROR", 1e-15f), par.scalar<float>("MAX_TOLERANCE", 1e-10f),
+                      par.scalar<int>("RELATIVE_ERRORS", 0), 1, par.scalar<float>("M1_SP_Fix_Factor", 0.f),
+                      par.scalar<float>("Tolerance_Fix_Factor", 0.f), par.scalar<float>("SPLRef", 2e-5f), 0, 0, 1,
+                      j_fast, j_slow, passes, 17, 2),
+       "BMOHCNewKernel");
+    bm_ms = cm_last_kernel_ms();
+    ok(BMOHCKernel_Copy_Results(speeds.data(), speeds.size(), 0), "BMOHCKernel_Copy_Results");
+    auto dev_copy = [&](void *src, size_t n_bytes, void *dst) {
+      ok(GeneralKernel_Copy_Results_Bytes(dst, src, n_bytes, 0), "GeneralKernel_Copy_Results_Bytes");
+    };
+    std::vector<int32_t> failed_nodes(bufP1), failed_blocks(TB);
+    dev_copy(getCudaFailedTimeNodes(), failed_nodes.size() * 4, failed_nodes.data());
+    dev_copy(getCudaFailedBlocks(), failed_blocks.size() * 4, failed_blocks.data());
+    std::vector<float> conv_times(bufP1), conv_times_b(TB), conv_loops(bufP1), conv_loops_b(TB), conv_j(bufP1),
+        conv_j_b(TB);
+    ok(extractConvergenceTimes(conv_times.data(), conv_times.size()), "extractConvergenceTimes");
+    dev_copy(getCudaConvergeSpeedBlocks(), TB * 4, conv_times_b.data());
+    dev_copy(getCudaConvergedTimeNodes(), bufP1 * 4, conv_loops.data());
+    dev_copy(getCudaConvergedBlocks(), TB * 4, conv_loops_b.data());
+    dev_copy(getCudaConvergedJacobyLoopsPerIteration(), bufP1 * 4, conv_j.data());
+    dev_copy(getCudaConvergedJacobyLoopsPerIterationBlocks(), TB * 4, conv_j_b.data());
+    // convergence and failure data exist only for a run from the input (src/solver.cpp:706-721)
+    if (conv_tests & 1) out.write(conv_times.data() + overlap, rows);
+    if (conv_tests & 2) out.write(conv_times_b.data(), TB);
+    out.write_cast(failed_nodes.data() + overlap, rows);
+    out.write_cast(failed_blocks.data(), TB);
+    if (conv_tests & 4) {
+      out.write(conv_loops.data() + overlap, rows);
+      out.write(conv_loops_b.data(), TB);
+    }
+    if (conv_tests & 8) {
+      out.write(conv_j.data() + overlap, rows);
+      out.write(conv_j_b.data(), TB);
+    }
+  } else if (run_stage == 1) {
+    // resume from BM velocity: [totalTimeNodes][256] uploaded into the BM
+    // buffer (src/solver.cpp:683-692)
+    const std::vector<float> v = stage_data(par, 0, (size_t)total * kSections);
+    std::copy(v.begin(), v.end(), speeds.begin());
+    ok(ReverseKernel_Copy_Results(speeds.data(), (size_t)total * kSections), "ReverseKernel_Copy_Results");
   }
   if (allowed & 1) out.write(speeds.data() + (size_t)overlap * kSections, rows * kSections);
 
@@ -334,7 +380,21 @@ int run(const ParFile &par, const std::string &exe_dir) {
       for (int i = 0; i < kSections; ++i) aihc[j * kSections + i] = A3[j];
     ok(loadAihc(aihc.data()), "loadAihc");
     const auto t0 = std::chrono::steady_clock::now();
-    ok(RunIHCKernel(nullptr, 0, 1, 0, 0), "RunIHCKernel");
+    if (run_stage < 2) {
+      ok(RunIHCKernel(nullptr, 0, 1, 0, 0), "RunIHCKernel");
+    } else {
+      // resume from lambda (src/AudioGramCreator.cpp:895-902): plane j of the
+      // stage data is rows [j*total, (j+1)*total); upload, then the
+      // spontaneous-rate fix-up, keeping max(spont, lambda) for the output
+      const size_t plane = (size_t)total * kSections;
+      const std::vector<float> lam_in = stage_data(par, 0, kLambdas * plane);
+      ok(ReverseKernel_Copy_Results_Float(lam_in.data(), static_cast<float *>(getCudaLambda()), 0,
+                                          (size_t)kLambdas * total, kSections),
+         "ReverseKernel_Copy_Results_Float");
+      ok(updateCUDALambdaArray(static_cast<float *>(getCudaLambda()), static_cast<float *>(getCudaBuffer()),
+                               (size_t)total, kSections, 0, 0, 1),
+         "updateCUDALambdaArray");
+    }
     ok(cm_device_synchronize(), "cm_device_synchronize");
     ihc_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
     std::vector<float> lam((size_t)kLambdas * total * kSections);

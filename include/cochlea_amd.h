@@ -154,8 +154,20 @@ int releaseBuffer(int Show_Run_Time);
  * the IHC stage is not written over the BM velocity buffer */
 int RunIHCKernel(float *host_backup, int Show_Run_Time, int save_lambda, int backup_stage,
                  int Decouple_Unified_IHC_Factor);
-/* replaces BMOHCKernel_Copy_Lambda (src/cochlea.cu:834-837) */
+/* replaces BMOHCKernel_Copy_Lambda (src/cochlea.cu:834-837); reads the saved-lambda
+ * buffer when this run filled it, else the lambda planes */
 int BMOHCKernel_Copy_Lambda(float *target, size_t lambdaNodes, int offset);
+/* replaces ReverseKernel_Copy_Results_Template<JNDFloat> (src/cochlea.cu:3691-3693,
+ * include/cochlea_common.h:188); cuda_target: getCudaLambda(), getCudaBuffer() or the
+ * BM velocity buffer */
+int ReverseKernel_Copy_Results_Float(const float *cpu_src, float *cuda_target, size_t start_time_node,
+                                     size_t time_nodes, int sections);
+/* replaces updateCUDALambdaArray<JNDFloat> (src/cochlea.cu:4512-4540,
+ * include/cochlea_common.h:191): lambda-stage resume fix-up on getCudaLambda() */
+int updateCUDALambdaArray(float *lambda_array, float *cuda_buffer, size_t calc_time_nodes, int sections,
+                          int Show_Run_Time, int Show_Device_Data, int cuda_buffer_update);
+/* batched lambda fix-up over device pointers [3][rows][256] (no reference counterpart) */
+int cm_fix_lambda_run(float *d_lambda, float *d_buffer, int64_t rows, const float *spont3);
 /* replace getCudaLambda / getCudaBuffer / getCudaMeanRate */
 void *getCudaLambda(void);
 void *getCudaBuffer(void);

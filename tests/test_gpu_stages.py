@@ -274,3 +274,23 @@ def test_bm_tolerance_mode_parity(oracle, tau, ws):
     _say(f"rel L2 {_rel(v, o['speeds']):.3e}, sweeps gpu {st[:, 2]} oracle {o['sweeps']}")
     np.testing.assert_array_equal(v, o["speeds"])
     np.testing.assert_array_equal(st[:, 2], o["sweeps"])
+
+
+def test_lambda_fix_up_kernel():
+    """cm_fix_lambda_run (CUDAFIXJND_Lambda): lambda -> max(0, lambda - spont_k),
+    buffer -> max(spont_k, lambda), per fibre group, in float32."""
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from cochlearmodel_amd import _lib, stages
+    _say("case: lambda fix-up over 3 x 1001 x 256")
+    rng = np.random.default_rng(3)
+    lam = rng.uniform(-5.0, 200.0, (3, 1001, 256)).astype(np.float32)
+    d_lam = torch.from_numpy(lam.copy()).cuda()
+    d_buf = torch.empty_like(d_lam)
+    sp = np.array(stages.SPONT_DEFAULT, np.float32)
+    _lib.check(_lib.lib().cm_fix_lambda_run(_lib.ptr(d_lam), _lib.ptr(d_buf), 1001, _lib.ptr(sp)))
+    torch.cuda.synchronize()
+    s = sp[:, None, None]
+    np.testing.assert_array_equal(d_lam.cpu().numpy(), np.maximum(np.float32(0), lam - s))
+    np.testing.assert_array_equal(d_buf.cpu().numpy(), np.maximum(s, lam))

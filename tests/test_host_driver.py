@@ -132,3 +132,68 @@ def test_voice_run_tolerance_mode(tmp_path, oracle, warm):
         rel = np.linalg.norm(v - ref) / np.linalg.norm(ref)
         print(f"learned warm start vs exact solve: rel L2 {rel:.2e}", flush=True)
         assert rel <= 2e-3
+
+
+def _voice_par(tmp_path, extra):
+    out = tmp_path / "stage.bin"
+    (tmp_path / "stage.par").write_text(
+        f"Fs = 44100\nTime_Blocks = 5\nTime_Block_Length = 0.02\nOverlap_Time = 0.005\nIHC_Vector = 8\n"
+        f"Allowed_Outputs = 15\nOutput_Result = {out}\n" + extra)
+    return tmp_path / "stage.par", out
+
+
+def test_resume_needs_stage_data(tmp_path):
+    par, _ = _voice_par(tmp_path, "Run_Stage_Calculation = 1\n")
+    r = subprocess.run([EXE, str(par)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "need data or file" in r.stderr
+
+
+@pytest.mark.gpu
+def test_resume_from_bm_velocity(tmp_path, oracle):
+    """Run_Stage_Calculation = 1: the BM stage is skipped, the given velocity is
+    uploaded, and the lambda planes equal the oracle's IHC/lambda of it."""
+    from cochlearmodel_amd import solver, stages
+    lay = solver.voice_layout(FS, 5, 0.02, 0.005)
+    TB, ov, tot = lay.time_blocks, lay.overlap_nodes, lay.total_nodes
+    print(f"case: cochlea_run resumed from BM velocity, {tot} rows", flush=True)
+    rng = np.random.default_rng(11)
+    t = np.arange(tot)[:, None] / FS
+    vel = (1e-4 * np.sin(2 * np.pi * 900.0 * t + np.linspace(0, 3, 256)[None, :]) *
+           np.exp(-np.linspace(0, 4, 256))[None, :] + 1e-6 * rng.standard_normal((tot, 256))).astype(np.float32)
+    f = tmp_path / "velocity.f32"
+    vel.tofile(f)
+    par, out = _voice_par(tmp_path, f"Run_Stage_Calculation = 1\nRun_Stage_File_name = {f}\n")
+    r = subprocess.run([EXE, str(par)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = np.fromfile(out, np.float32)
+    b, _ = stages.load_ac_filter(FS)
+    lo = oracle.ihc_lambda(vel, tot, lay.block_nodes * TB, b, stages.dc_filter(FS), stages.ihc_damage(8.0),
+                           stages.aihc_table(), np.array(stages.SPONT_DEFAULT, np.float32), threads=8)
+    expect = np.concatenate([vel[ov:tot].ravel()] + [lo[j, ov:tot].ravel() for j in range(3)])
+    assert got.size == expect.size + 2
+    np.testing.assert_array_equal(got[:-2], expect)
+    assert got[-2] == 0.0                      # no BM kernel ran
+
+
+@pytest.mark.gpu
+def test_resume_from_lambda(tmp_path):
+    """Run_Stage_Calculation = 2: lambda planes are uploaded and fixed up
+    (CUDAFIXJND_Lambda); the written planes are the kept buffer
+    max(spont, lambda), the velocity stage is empty."""
+    from cochlearmodel_amd import solver, stages
+    lay = solver.voice_layout(FS, 5, 0.02, 0.005)
+    ov, tot = lay.overlap_nodes, lay.total_nodes
+    print(f"case: cochlea_run resumed from lambda, {tot} rows", flush=True)
+    rng = np.random.default_rng(12)
+    lam = rng.uniform(0.0, 120.0, (3, tot, 256)).astype(np.float32)
+    f = tmp_path / "lambda.f32"
+    lam.tofile(f)
+    par, out = _voice_par(tmp_path, f"Run_Stage_Calculation = 2\nRun_Stage_File_name = {f}\n")
+    r = subprocess.run([EXE, str(par)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = np.fromfile(out, np.float32)
+    sp = np.array(stages.SPONT_DEFAULT, np.float32)[:, None, None]
+    kept = np.maximum(sp, lam)
+    expect = np.concatenate([np.zeros((tot - ov) * 256, np.float32)] + [kept[j, ov:tot].ravel() for j in range(3)])
+    assert got.size == expect.size + 2
+    np.testing.assert_array_equal(got[:-2], expect)
