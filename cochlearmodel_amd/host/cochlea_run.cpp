@@ -32,6 +32,13 @@
 // 2: from lambda, three such planes) with Run_Stage_File_name (raw float32) or
 // Run_Stage_Vector: the stages before it are skipped, as are the convergence
 // and failure vectors, which only a run from the input produces.
+// JND runs (Sim_Type 3 or 4 with Calculate_JND = 1): testedFrequencies,
+// testedPowerLevels, testedNoises, Generate_One_Ref_Per_Noise_Level,
+// Complex_Profile_* vectors, JND_Interval_Duration/Head/Tail,
+// JND_Delta_Alpha_Time_Factor, Calculate_JND_Types, Calculate_From_Mean_Rate,
+// Complex_JND_Calculation_Types, Approximated_JND_EPS_Diff, Max_dB_SPL_JND, W,
+// Input_Noise / Input_Noise_File, JND_File_Name, JND_Raw_File_Name.  Noise_Seed
+// (no reference counterpart) seeds the white noise when none is given.
 // Keys of this library's solver options (no reference counterpart):
 // Jacobi_Tolerance (> 0: Jacobi to that relative residual instead of the fixed
 // sweeps), Jacobi_Check_Every (4), Jacobi_Max_Sweeps (1000), Warm_Start (0
@@ -42,6 +49,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdint>
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
@@ -413,6 +421,290 @@ int run(const ParFile &par, const std::string &exe_dir) {
   return 0;
 }
 
+// --------------------------------------------------------- JND runs ---
+// Profile runs with Calculate_JND (Sim_Type 3) and complex-JND runs (Sim_Type
+// 4): CParams::updateInputProfile (src/params.cpp:661-782) builds one interval
+// per (noise, level, frequency) plus the reference intervals; then input
+// synthesis, BM, IHC/lambda and the JND reductions run on the device, and the
+// final dB, failed-signal counts and (Sim_Type 4) the complex-JND aggregation
+// on the host (src/solver.cpp:735-835, src/AudioGramCreator.cpp:454-571).
+struct JndLayout {
+  std::vector<cm_jnd_profile> prof;
+  std::vector<int> refs, serial, calc, to_ref;
+};
+
+JndLayout jnd_layout(const std::vector<float> &freqs, const std::vector<float> &levels,
+                     const std::vector<float> &noises, bool one_ref_per_noise, double spl_ref,
+                     const std::vector<double> &divisor, const std::vector<double> &fix_factor,
+                     const std::vector<double> &fix_add) {
+  constexpr float kNoNoise = 1111.f;   // MIN_INF_POWER_LEVEL
+  auto spl2pa = [&](double db) { return db == kNoNoise ? 0.0 : 10.0 * std::pow(10.0, db / 20.0) * spl_ref; };
+  const int tf = (int)freqs.size(), tp = (int)levels.size(), tn = (int)noises.size();
+  if (tf == 0 || tp == 0 || tn == 0) throw std::runtime_error("testedFrequencies/PowerLevels/Noises must not be empty");
+  JndLayout L;
+  const int add_f = one_ref_per_noise ? 0 : 1, add_n = 1 - add_f;
+  L.prof.assign((size_t)((tf + add_f) * tp + add_n) * tn, cm_jnd_profile{});
+  int g = 0, calc = 0;
+  auto reference = [&](int nl) {
+    cm_jnd_profile &r = L.prof[g];
+    r = cm_jnd_profile{};
+    r.dBSPLNoise = noises[nl] != kNoNoise ? noises[nl] : 0.f;
+    r.Wn = spl2pa(noises[nl]);
+    r.isReference = 1;
+    r.referenceIndex = g;
+    r.calculateIndexPosition = -1;
+    L.refs.push_back(g++);
+  };
+  for (int nl = 0; nl < tn; ++nl) {
+    for (int pl = 0; pl < tp; ++pl) {
+      for (int fl = 0; fl < tf; ++fl) {
+        const size_t cid = (size_t)tf * nl + fl;
+        float base = levels[pl];
+        if (cid < divisor.size() && divisor[cid] != 0 && noises[nl] != kNoNoise && noises[nl] > 0)
+          base = levels[0] + (levels[pl] - levels[0]) * (float)divisor[cid] / noises[nl];
+        if (noises[nl] != kNoNoise)
+          base += (float)(cid < fix_factor.size() ? fix_factor[cid] : 0.0) * noises[nl] +
+                  (float)(cid < fix_add.size() ? fix_add[cid] : 0.0);
+        cm_jnd_profile &p = L.prof[g];
+        p = cm_jnd_profile{};
+        const double pw = spl2pa(base);
+        p.dBSPLSignal = base;
+        p.dBSPLNoise = noises[nl] != kNoNoise ? noises[nl] : 0.f;
+        p.referenceIndex = one_ref_per_noise ? tp * tf * (nl + 1) + nl : g + tf - fl;
+        p.calculateIndexPosition = calc++;
+        p.dA = pw;
+        p.recipdA = pw > 0 ? 1.0 / pw : 0.0;
+        p.Wn = spl2pa(noises[nl]);
+        p.frequency = freqs[fl];
+        ++g;
+      }
+      if (!one_ref_per_noise) reference(nl);
+    }
+    if (one_ref_per_noise) reference(nl);
+  }
+  // JND_Serial_Intervals_Positions / JND_Interval_To_Reference (src/params.cpp:1596-1616)
+  const int n = (int)L.prof.size(), nr = (int)L.refs.size();
+  L.serial.assign(n, 0);
+  for (int i = 0; i < n; ++i) {
+    const int rr = (int)((int64_t)i * nr / n);
+    if (std::find(L.refs.begin(), L.refs.end(), i) == L.refs.end()) {
+      L.serial[i] = (int)L.calc.size();
+      L.calc.push_back(i);
+      L.to_ref.push_back(L.refs[rr]);
+    } else {
+      L.serial[i] = rr;
+    }
+  }
+  return L;
+}
+
+// transposeVector (include/cvector.cpp:520-530)
+std::vector<float> transpose(const std::vector<float> &v, size_t column_size) {
+  std::vector<float> r(v.size());
+  const size_t row = v.size() / column_size;
+  for (size_t i = 0; i < v.size(); ++i) r[(i % row) * column_size + i / row] = v[i];
+  return r;
+}
+
+// white noise of the profile runs: Input_Noise, Input_Noise_File (raw float64)
+// or, in place of the reference's random_device draw, N(0,1) from a seeded
+// 64-bit LCG with Box-Muller (Noise_Seed, default 0)
+std::vector<float> jnd_noise(const ParFile &par, int n) {
+  std::vector<double> src;
+  if (par.has("Input_Noise")) src = par.vec<double>("Input_Noise");
+  else if (!par.str("Input_Noise_File").empty()) src = read_float64_file(par.str("Input_Noise_File"));
+  std::vector<float> wn(n, 0.f);
+  if (!src.empty()) {
+    for (int i = 0; i < n && i < (int)src.size(); ++i) wn[i] = (float)src[i];
+    return wn;
+  }
+  uint64_t st = 0x9E3779B97F4A7C15ull ^ (uint64_t)par.scalar<long long>("Noise_Seed", 0);
+  auto uni = [&]() {
+    st = st * 6364136223846793005ull + 1442695040888963407ull;
+    return ((st >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+  };
+  for (int i = 0; i < n; i += 2) {
+    const double r = std::sqrt(-2.0 * std::log(uni())), t = 2.0 * M_PI * uni();
+    wn[i] = (float)(r * std::cos(t));
+    if (i + 1 < n) wn[i + 1] = (float)(r * std::sin(t));
+  }
+  return wn;
+}
+
+int run_jnd(const ParFile &par, const std::string &exe_dir) {
+  const int sim_type = par.scalar<int>("Sim_Type", 0);
+  const float Fs = par.scalar<float>("Fs", 44100.f);
+  const double spl_ref = par.scalar<double>("SPLRef", 2e-5);
+  const std::vector<float> freqs = par.has("testedFrequencies")
+                                       ? par.vec<float>("testedFrequencies")
+                                       : std::vector<float>{250.f, 500.f, 1000.f, 2000.f, 3000.f, 4000.f, 6000.f};
+  const std::vector<float> levels = par.has("testedPowerLevels") ? par.vec<float>("testedPowerLevels")
+                                                                 : std::vector<float>{10.f};
+  const std::vector<float> noises = par.has("testedNoises") ? par.vec<float>("testedNoises")
+                                                            : std::vector<float>{1111.f};
+  const bool one_ref = par.scalar<int>("Generate_One_Ref_Per_Noise_Level", 1) == 1;
+  const JndLayout lay = jnd_layout(freqs, levels, noises, one_ref, spl_ref,
+                                   par.vec<double>("Complex_Profile_Power_Level_Divisor"),
+                                   par.vec<double>("Complex_Profile_Noise_Level_Fix_Factor"),
+                                   par.vec<double>("Complex_Profile_Noise_Level_Fix_Addition"));
+  const std::string jnd_file = par.str("JND_File_Name");
+  if (jnd_file.empty()) throw std::runtime_error("JND File is missing, cannot output results...");
+  const float dur = par.scalar<float>("JND_Interval_Duration", 0.02f);
+  const float head_s = par.scalar<float>("JND_Interval_Head", 0.014f);
+  const float tail_s = par.scalar<float>("JND_Interval_Tail", 0.002f);
+  const double dalpha = par.scalar<double>("JND_Delta_Alpha_Time_Factor", 0.2);
+  const int interval = (int)std::lround(Fs * dur);          // JND_Interval_Nodes_Full (roundf)
+  const int head = (int)std::lround(Fs * head_s);
+  const int length = (int)std::lround(Fs * (dur - head_s - tail_s));
+  const float block_len = par.scalar<float>("Time_Block_Length", 0.02f);
+  const int block = (int)std::ceil(block_len * Fs);
+  const int DF = par.scalar<int>("Decouple_Filter", interval / block);
+  if (DF <= 0 || DF * block != interval)
+    throw std::runtime_error("a profile run needs JND_Interval_Duration = Decouple_Filter x Time_Block_Length");
+  const int n_prof = (int)lay.prof.size(), n_calc = (int)lay.calc.size();
+  const int TB = n_prof * DF;
+  const long overlap_us = (long)std::llround(par.scalar<double>("Overlap_Time", 0.005) * 1000.0) * 1000;
+  const int overlap = (int)(overlap_us * (double)Fs / 1000000.0);
+  const int bufP1 = overlap + block * (TB + 1);
+  const int rows = n_prof * interval;
+  const unsigned types = par.scalar<unsigned>("Calculate_JND_Types", 0xFFFFu);
+  const int calc_rms = (types & 1) != 0, calc_ai = (types & 2) != 0;
+  std::printf("cochlea_run: Sim_Type %d, %d intervals (%d calculated), %d time blocks\n", sim_type, n_prof, n_calc, TB);
+
+  // model and buffers
+  std::vector<float> sec(9 * kSections);
+  ok(cm_pack_section_params(sec.data()), "cm_pack_section_params");
+  std::vector<double> lo(kSections), mid(kSections), up(kSections);
+  ok(cm_tri_matrix(lo.data(), mid.data(), up.data()), "cm_tri_matrix");
+  std::vector<float> upf(up.begin(), up.end()), lof(lo.begin(), lo.end());
+  std::vector<double> ohc = par.has("OHC_Vector") ? par.vec<double>("OHC_Vector") : std::vector<double>{0.5};
+  const std::vector<float> gamma = expand(ohc, kSections);
+  const float *S = sec.data();
+  const size_t in_nodes = (size_t)bufP1 + 2 * overlap;
+  ok(BMOHCKernel_Init(gamma.data(), S + 0 * kSections, S + 1 * kSections, upf.data(), lof.data(), S + 2 * kSections,
+                      S + 3 * kSections, S + 4 * kSections, S + 5 * kSections, S + 6 * kSections, S + 7 * kSections,
+                      1.f, 0.f, in_nodes, in_nodes * kSections, 0, 1, 0, 0),
+     "BMOHCKernel_Init");
+
+  // input synthesis into the BM input buffer (generateInputFromProfile)
+  const std::vector<float> wn = jnd_noise(par, interval);
+  const int e1 = (int)((dur - tail_s) * Fs);
+  ok(InputProfilesArrayInitialization(n_prof, interval, 0, 0, 0), "InputProfilesArrayInitialization");
+  const float one = 1.f;
+  ok(generateInputFromProfile(lay.prof.data(), wn.data(), interval, nullptr, 0, 0, 0, 0,
+                              par.scalar<double>("Normalize_Noise_Energy_To_Given_Interval", 0.16), 0.0, 0, e1, 0, e1,
+                              n_prof, 0, 0, n_prof, overlap, interval, head, length, Fs, 0, nullptr, 0, 1,
+                              par.scalar<float>("MAX_M1_SP_ERROR", 1e-15f), par.scalar<float>("MAX_TOLERANCE", 1e-10f),
+                              0, 0.f, 0.f, DF, 0, &one, 1, &one, 1),
+     "generateInputFromProfile");
+  cm_bm_scalars sc;
+  ok(cm_default_bm_scalars(Fs, &sc), "cm_default_bm_scalars");
+  ok(cm_set_bm_solver(0, 1e-4f, 4, 1000, 0), "cm_set_bm_solver");
+  ok(BMOHCNewKernel(nullptr, 1, sc.w_ohc, sc.time_step, sc.time_step_out, sc.delta_x, sc.alpha_r, 1, 1, 0, sc.Ts,
+                    sc.alpha_l, sc.alpha_s, sc.Gme, sc.a0, sc.a1, sc.a2, sc.sigma_ow, sc.eta1, sc.eta2, TB, bufP1,
+                    overlap, overlap_us, 1, sc.dt_max, sc.dt_min, DF, par.scalar<float>("MAX_M1_SP_ERROR", 1e-15f),
+                    par.scalar<float>("MAX_TOLERANCE", 1e-10f), 0, 0, 0.f, 0.f, (float)spl_ref, 0, 0, 0,
+                    par.scalar<int>("JACOBBY_Loops_Fast", 2), par.scalar<int>("JACOBBY_Loops_Slow", 20),
+                    par.scalar<int>("Cuda_Outern_Loops", 10), 17, 2),
+     "BMOHCNewKernel");
+  const float bm_ms = cm_last_kernel_ms();
+  std::vector<int32_t> failed_blocks(TB);
+  ok(GeneralKernel_Copy_Results_Bytes(failed_blocks.data(), getCudaFailedBlocks(), TB * 4, 0), "failed blocks");
+
+  // IHC / lambda over the decoupled intervals
+  std::vector<double> fm2 = par.has("AC_Filter_Vector")
+                                ? par.vec<double>("AC_Filter_Vector")
+                                : read_filter_fixture(exe_dir + "/data/ac_filter_equiripple_" +
+                                                      std::to_string((int)Fs) + ".json");
+  const int order = (int)fm2.at(0);
+  const bool fir = (int)fm2.at(1) == 1;
+  std::vector<double> b, a;
+  if (fir) {
+    b.assign(fm2.begin() + 2, fm2.begin() + 2 + order);
+  } else {
+    a.assign(fm2.begin() + 2, fm2.begin() + 2 + order);
+    b.assign(fm2.begin() + 2 + order, fm2.begin() + 2 + 2 * order);
+  }
+  const int ndc = (int)std::lround(Fs * 2e-3f);
+  std::vector<double> dc(ndc, (double)(1.0f / (Fs * 2e-3f)));
+  std::vector<double> ihcv = par.has("IHC_Vector") ? par.vec<double>("IHC_Vector") : std::vector<double>{8.0};
+  const std::vector<float> ihc = expand(ihcv, kSections);
+  std::vector<double> dmg(kSections);
+  for (int i = 0; i < kSections; ++i) dmg[i] = std::pow(10.0, (double)ihc[i]);
+  std::vector<float> W = par.has("W") ? par.vec<float>("W") : std::vector<float>{0.61f, 0.23f, 0.16f};
+  W.resize(3, 0.f);
+  const float nc[9] = {60.f, 3.f, 0.1f, 0.f, 0.f, 0.f, W[0], W[1], W[2]};
+  ok(IHCNewKernel(dmg.data(), nc, dc.data(), ndc, b.data(), fir ? nullptr : a.data(), fir ? 0 : 1, (int)b.size(),
+                  kSections, TB, spl_ref, nullptr, 0, rows, rows, rows, block, 0, 0,
+                  par.scalar<float>("Lambda_SAT", 500.f), 1.f, 100.f, 1, 1, 0, 0, 0, 0, 1.f, 1, 1, 0, DF, 0),
+     "IHCNewKernel");
+  ok(enableloadAihc(), "enableloadAihc");
+  std::vector<float> aihc(kLambdas * kSections);
+  const float A3[3] = {70.f, 50.f, 30.f};
+  for (int j = 0; j < kLambdas; ++j)
+    for (int i = 0; i < kSections; ++i) aihc[j * kSections + i] = A3[j];
+  ok(loadAihc(aihc.data()), "loadAihc");
+  ok(RunIHCKernel(nullptr, 0, 0, 0, 0), "RunIHCKernel");
+
+  // JND reductions, final dB, failed signals
+  std::vector<double> nIHC(kSections, 30000.0 / kSections);
+  std::vector<float> F_RA(3 * (size_t)n_prof, 0.f), FAI(3 * (size_t)n_prof, 0.f);
+  ok(CudaCalculateJND(calc_ai, calc_rms, 0, 3 * n_prof, spl_ref, Fs, 1.0, nIHC.data(), lay.calc.data(), n_calc,
+                      lay.refs.data(), (int)lay.refs.size(), 0, n_prof, n_prof, head, overlap, interval, length,
+                      lay.serial.data(), lay.to_ref.data(), F_RA.data(), FAI.data(), rows * kSections,
+                      rows * kSections, dalpha == 0 ? (double)length : dalpha * Fs, 1, 0, 0, 0, 0),
+     "CudaCalculateJND");
+  std::vector<double> ai(n_prof, 0.0), rate(n_prof, 0.0);
+  ok(cm_jnd_final(F_RA.data(), FAI.data(), n_prof, n_prof, 0, lay.refs.data(), (int)lay.refs.size(),
+                  lay.serial.data(), spl_ref, par.scalar<double>("Max_dB_SPL_JND", 150.0), calc_ai, calc_rms,
+                  ai.data(), rate.data(), nullptr, nullptr),
+     "cm_jnd_final");
+  std::vector<int32_t> failed_signals(n_prof, 0);
+  ok(cm_jnd_failed_signals(failed_blocks.data(), TB, n_prof, 0, lay.refs.data(), (int)lay.refs.size(),
+                           lay.serial.data(), DF, failed_signals.data()),
+     "cm_jnd_failed_signals");
+
+  // raw JND output (src/solver.cpp:756-806)
+  const std::string raw_name = par.str("JND_Raw_File_Name");
+  const bool write_raw = sim_type != 4 || !raw_name.empty();
+  if (write_raw) {
+    BinWriter raw(sim_type == 4 ? raw_name : jnd_file);
+    const size_t mcols = (size_t)(n_calc / (levels.size() * freqs.size())) * levels.size();
+    raw.write_cast(failed_signals.data(), n_calc);
+    for (int pass = 0; pass < 2; ++pass) {
+      if ((pass == 0 && !calc_ai) || (pass == 1 && !calc_rms)) continue;
+      const std::vector<double> &src = pass == 0 ? ai : rate;
+      std::vector<float> v(src.begin(), src.begin() + n_calc);
+      raw.write(v.data(), v.size());                 // jnd_ai_power / jnd_rms_power
+      const std::vector<float> t = transpose(v, mcols);
+      raw.write(t.data(), t.size());                 // jnd_ai / jnd_rms
+    }
+  }
+  // complex JND (Sim_Type 4, src/solver.cpp:809-835)
+  if (sim_type == 4) {
+    const unsigned methods = par.scalar<unsigned>("Complex_JND_Calculation_Types", 1u) & 3u;
+    const int n_methods = (methods & 1) + ((methods >> 1) & 1);
+    const int n_cplx = (int)(freqs.size() * noises.size());
+    std::vector<int32_t> iv((size_t)n_cplx * levels.size());
+    ok(cm_complex_jnd_intervals((int)freqs.size(), (int)levels.size(), (int)noises.size(), iv.data()),
+       "cm_complex_jnd_intervals");
+    const std::vector<double> &src = par.scalar<int>("Calculate_From_Mean_Rate", 1) == 1 ? rate : ai;
+    std::vector<float> fin((size_t)n_methods * n_cplx);
+    std::vector<int32_t> warn(fin.size()), conv(fin.size());
+    ok(cm_complex_jnd(src.data(), failed_signals.data(), n_calc, iv.data(), n_cplx, (int)levels.size(), (int)methods,
+                      par.scalar<float>("Approximated_JND_EPS_Diff", 1.f), fin.data(), warn.data(), conv.data()),
+       "cm_complex_jnd");
+    BinWriter fo(jnd_file);
+    fo.write(fin.data(), fin.size());                // jnd_final
+    fo.write_cast(warn.data(), warn.size());         // jnd_final_warnings
+    fo.write_cast(conv.data(), conv.size());         // jnd_final_signal_convergence_warning
+  }
+  std::printf("cochlea_run: JND written to %s (BM %.2f ms)\n", jnd_file.c_str(), bm_ms);
+  ok(BMOHCKernel_Free(), "BMOHCKernel_Free");
+  ok(IHCKernel_Free(), "IHCKernel_Free");
+  return 0;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -427,6 +719,9 @@ int main(int argc, char **argv) {
       return 2;
     }
     ParFile par(argv[1]);
+    const int sim_type = par.scalar<int>("Sim_Type", 0);
+    if ((sim_type == 3 || sim_type == 4) && par.scalar<int>("Calculate_JND", 0) == 1)
+      return run_jnd(par, dir_of(argv[0]));
     return run(par, dir_of(argv[0]));
   } catch (const std::exception &e) {
     std::fprintf(stderr, "cochlea_run: %s\n", e.what());

@@ -197,3 +197,53 @@ def test_resume_from_lambda(tmp_path):
     expect = np.concatenate([np.zeros((tot - ov) * 256, np.float32)] + [kept[j, ov:tot].ravel() for j in range(3)])
     assert got.size == expect.size + 2
     np.testing.assert_array_equal(got[:-2], expect)
+
+
+def test_jnd_run_needs_output_file(tmp_path):
+    p = tmp_path / "jnd.par"
+    p.write_text("Sim_Type = 4\nCalculate_JND = 1\ntestedFrequencies = 1000\ntestedPowerLevels = 20 40\n")
+    r = subprocess.run([EXE, str(p)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "JND File is missing" in r.stderr
+
+
+@pytest.mark.gpu
+def test_complex_jnd_run_files(tmp_path):
+    """Sim_Type 4 through cochlea_run: the raw JND file (failed signals, AI and
+    rate JND, each also transposed frequency-major) and the final file
+    (complex JND per method and profile, warnings) against the same sweep run
+    through the library's batched entry points."""
+    from cochlearmodel_amd import solver
+    from cochlearmodel_amd.jnd_sweep import JndSweep
+    print("case: cochlea_run Sim_Type 4, 2 freq x 3 levels x 2 noises, 40 ms intervals", flush=True)
+    sw = JndSweep((1000.0, 3000.0), (20.0, 40.0, 60.0), (1111.0, 40.0), duration=0.04, methods=3)
+    sw.step(solver.default_scalars(FS), timed_bm=False)
+    res = sw.finish()
+    wn = tmp_path / "noise.f64"
+    sw.WN.astype(np.float64).tofile(wn)
+    fin, raw = tmp_path / "jnd_final.bin", tmp_path / "jnd_raw.bin"
+    (tmp_path / "jnd.par").write_text(
+        "Sim_Type = 4\nCalculate_JND = 1\nFs = 44100\ntestedFrequencies = 1000 3000\ntestedPowerLevels = 20 40 60\n"
+        f"testedNoises = 1111 40\nJND_Interval_Duration = 0.04\nComplex_JND_Calculation_Types = 3\n"
+        f"Input_Noise_File = {wn}\nJND_File_Name = {fin}\nJND_Raw_File_Name = {raw}\n")
+    r = subprocess.run([EXE, str(tmp_path / "jnd.par")], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    n = len(sw.calc)
+    got_raw = np.fromfile(raw, np.float32)
+    assert got_raw.size == 5 * n
+    fs_, ai, ai_t, rate, rate_t = (got_raw[k * n:(k + 1) * n] for k in range(5))
+    np.testing.assert_array_equal(fs_, res["failed_signals"].astype(np.float32))
+    d_ai = np.abs(ai - res["ai_db"].astype(np.float32)).max()
+    d_rate = np.abs(rate - res["rate_db"].astype(np.float32)).max()
+    print(f"driver vs batched sweep: AI {d_ai:.2e} dB, rate {d_rate:.2e} dB", flush=True)
+    assert d_ai <= 1e-3 and d_rate <= 1e-3
+    # calc index (noise, level, freq) -> frequency-major columns of noise x level
+    nf, nl, nn = 2, 3, 2
+    for i in range(n):
+        f, rest = i % nf, i // nf
+        assert ai_t[f * (nn * nl) + rest] == ai[i] and rate_t[f * (nn * nl) + rest] == rate[i]
+    got_fin = np.fromfile(fin, np.float32)
+    val, warn, conv = res["complex"]
+    assert got_fin.size == 3 * val.size
+    np.testing.assert_allclose(got_fin[:val.size], val.reshape(-1), atol=1e-3)
+    np.testing.assert_array_equal(got_fin[val.size:2 * val.size], warn.reshape(-1).astype(np.float32))
+    np.testing.assert_array_equal(got_fin[2 * val.size:], conv.reshape(-1).astype(np.float32))
