@@ -1,0 +1,324 @@
+// AC (low-pass) filter design of the IHC stage, host C++.
+//
+// The reference designs the AC filter at run time from its Function_Filter
+// string (SubModel::analyzeACFilter, src/SubModel.cpp:153-360) with the firpm
+// library (src/firpm/*, GPL, needs Eigen).  This file has its own
+// Parks-McClellan design: the Remez exchange on a dense frequency grid
+// (McClellan, Parks & Rabiner, "A computer program for designing optimum FIR
+// linear phase digital filters", IEEE Trans. Audio Electroacoust. 21(6), 1973:
+// grid of 16 points per basis function, barycentric Lagrange interpolation in
+// x = cos(2 pi f), alternation exchange, coefficients by an inverse DFT of the
+// amplitude response), and around it the reference's equiripple procedure:
+// minimum order (Oppenheim & Schafer eq. 7.104, SubModel::calcMinimumOrder),
+// band weights from the ripples, DC-gain normalisation (fixDCGain), the -Apass
+// corner correction (FIRFreqError, src/firpm/pm.cpp:742-793) and a redesign,
+// then unit FIR gain (HFunction::setFIRGain(0), src/HFunction.cpp:43-46).
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <cstdlib>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "cm_internal.h"
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+
+struct Grid {
+  std::vector<double> f, x, des, wt;
+};
+
+// Dense grid over the pass band [0, fp] and the stop band [fs, 0.5] (cycles per
+// sample): steps of 0.5 / (density * nfcns) from each band's lower edge, the
+// last point of a band moved onto its upper edge.  For even lengths the
+// amplitude carries a cos(pi f) factor: desired / cos, weight * cos, and no
+// point at the Nyquist frequency where that factor vanishes.
+Grid make_grid(int ntaps, int nfcns, double fp, double fs, double wpass, double wstop, int density) {
+  Grid g;
+  const double delf = 0.5 / (density * (double)nfcns);
+  const double edges[2][2] = {{0.0, fp}, {fs, 0.5}};
+  const double des[2] = {1.0, 0.0}, wt[2] = {wpass, wstop};
+  for (int b = 0; b < 2; ++b) {
+    const double lo = edges[b][0], hi = edges[b][1];
+    const size_t first = g.f.size();
+    for (int k = 0;; ++k) {
+      const double f = lo + k * delf;
+      if (f > hi) break;
+      g.f.push_back(f);
+    }
+    if (g.f.size() == first) g.f.push_back(hi);
+    g.f.back() = hi;
+    for (size_t i = first; i < g.f.size(); ++i) {
+      g.des.push_back(des[b]);
+      g.wt.push_back(wt[b]);
+    }
+  }
+  const bool even = ntaps % 2 == 0;
+  if (even && g.f.back() > 0.5 - delf) {
+    g.f.pop_back();
+    g.des.pop_back();
+    g.wt.pop_back();
+  }
+  g.x.resize(g.f.size());
+  for (size_t i = 0; i < g.f.size(); ++i) {
+    g.x[i] = std::cos(2.0 * kPi * g.f[i]);
+    if (even) {
+      const double c = std::cos(kPi * g.f[i]);
+      g.des[i] /= c;
+      g.wt[i] *= c;
+    }
+  }
+  return g;
+}
+
+// barycentric weights 1 / prod_{j != i} 2 (x_i - x_j) of the nodes x[0..n)
+std::vector<double> bary_weights(const std::vector<double> &x, int n) {
+  std::vector<double> w(n);
+  for (int i = 0; i < n; ++i) {
+    double p = 1.0;
+    for (int j = 0; j < n; ++j)
+      if (j != i) p *= 2.0 * (x[i] - x[j]);
+    w[i] = 1.0 / p;
+  }
+  return w;
+}
+
+struct Interp {
+  std::vector<double> x, c, w;   // nodes, values, barycentric weights
+  double eval(double xv) const {
+    double num = 0.0, den = 0.0;
+    for (size_t i = 0; i < x.size(); ++i) {
+      const double d = xv - x[i];
+      if (d == 0.0) return c[i];
+      const double t = w[i] / d;
+      num += t * c[i];
+      den += t;
+    }
+    return num / den;
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+// Parks-McClellan low-pass of ntaps coefficients: pass band [0, fpass], stop
+// band [fstop, 0.5] in cycles per sample, weights wpass / wstop, grid density
+// (16 in the classic program).  h: ntaps outputs; delta (may be NULL): the
+// final weighted ripple; iterations (may be NULL).
+int cm_remez_lowpass(int ntaps, double fpass, double fstop, double wpass, double wstop, int density, double *h,
+                     double *delta, int *iterations) {
+  if (ntaps < 3 || ntaps > 4096 || !h || !(fpass > 0.0) || !(fstop > fpass) || !(fstop < 0.5) || !(wpass > 0.0) ||
+      !(wstop > 0.0) || density < 1) {
+    cm::set_error("cm_remez_lowpass: bad arguments (ntaps %d, fpass %g, fstop %g)", ntaps, fpass, fstop);
+    return -3;
+  }
+  const bool even = ntaps % 2 == 0;
+  const int nfcns = even ? ntaps / 2 : ntaps / 2 + 1;   // cosine basis functions
+  const int r = nfcns + 1;                              // extremal points
+  const Grid g = make_grid(ntaps, nfcns, fpass, fstop, wpass, wstop, density);
+  const int ng = (int)g.f.size();
+  if (ng < 2 * r) {
+    cm::set_error("cm_remez_lowpass: grid of %d points too small for %d extremals", ng, r);
+    return -3;
+  }
+  std::vector<int> ext(r);
+  for (int j = 0; j < nfcns; ++j) ext[j] = (int)((int64_t)j * (ng - 1) / nfcns);
+  ext[nfcns] = ng - 1;
+  std::vector<double> err(ng);
+  Interp P;
+  double dev = 0.0;
+  int it = 0;
+  for (; it < 250; ++it) {
+    // deviation and interpolant through the current extremal set
+    std::vector<double> xe(r);
+    for (int i = 0; i < r; ++i) xe[i] = g.x[ext[i]];
+    const std::vector<double> gam = bary_weights(xe, r);
+    double num = 0.0, den = 0.0;
+    for (int i = 0; i < r; ++i) {
+      num += gam[i] * g.des[ext[i]];
+      den += ((i & 1) ? -1.0 : 1.0) * gam[i] / g.wt[ext[i]];
+    }
+    dev = num / den;
+    P.x.assign(xe.begin(), xe.begin() + nfcns);
+    P.c.resize(nfcns);
+    for (int i = 0; i < nfcns; ++i) P.c[i] = g.des[ext[i]] - ((i & 1) ? -1.0 : 1.0) * dev / g.wt[ext[i]];
+    P.w = bary_weights(P.x, nfcns);
+    for (int j = 0; j < ng; ++j) err[j] = g.wt[j] * (g.des[j] - P.eval(g.x[j]));
+    // local extrema of the weighted error with |E| >= |dev|, alternating in sign
+    // (the error at a reference point equals |dev| only up to the rounding of
+    // D - (D - dev / W), hence the slack)
+    const double floor_abs = std::fabs(dev) * (1.0 - 1e-9) - 1e-14;
+    std::vector<int> cand;
+    for (int j = 0; j < ng; ++j) {
+      const double e = err[j];
+      if (std::fabs(e) < floor_abs) continue;
+      const bool ge_l = j == 0 || (e > 0 ? e >= err[j - 1] : e <= err[j - 1]);
+      const bool ge_r = j == ng - 1 || (e > 0 ? e >= err[j + 1] : e <= err[j + 1]);
+      if (ge_l && ge_r) cand.push_back(j);
+    }
+    std::vector<int> alt;
+    for (int j : cand) {
+      if (!alt.empty() && (err[alt.back()] > 0) == (err[j] > 0)) {
+        if (std::fabs(err[j]) > std::fabs(err[alt.back()])) alt.back() = j;
+      } else {
+        alt.push_back(j);
+      }
+    }
+    while ((int)alt.size() > r) {
+      if (std::fabs(err[alt.front()]) < std::fabs(err[alt.back()])) alt.erase(alt.begin());
+      else alt.pop_back();
+    }
+    if ((int)alt.size() < r) break;   // the current set is already the best alternation on this grid
+    double emax = 0.0, emin = 1e300;
+    for (int j : alt) {
+      emax = std::max(emax, std::fabs(err[j]));
+      emin = std::min(emin, std::fabs(err[j]));
+    }
+    const bool same = alt == ext;
+    ext = alt;
+    if (same || (emax - emin) <= 1e-13 * emax) {
+      ++it;
+      break;
+    }
+  }
+  // amplitude at f = m / ntaps, then the inverse DFT of a linear-phase response
+  const int half = (ntaps - 1) / 2;
+  std::vector<double> A(half + 1);
+  for (int m = 0; m <= half; ++m) {
+    const double f = (double)m / ntaps;
+    double a = P.eval(std::cos(2.0 * kPi * f));
+    if (even) a *= std::cos(kPi * f);
+    A[m] = a;
+  }
+  const double mid = 0.5 * (ntaps - 1);
+  for (int n = 0; n < ntaps; ++n) {
+    double s = A[0];
+    for (int m = 1; m <= half; ++m) s += 2.0 * A[m] * std::cos(2.0 * kPi * m * (n - mid) / ntaps);
+    h[n] = s / ntaps;
+  }
+  if (delta) *delta = std::fabs(dev);
+  if (iterations) *iterations = it;
+  return 0;
+}
+
+// |H(e^{j pi omega})| of an FIR, omega in [0, 1] (Nyquist = 1)
+static double fir_gain(const std::vector<double> &h, double omega) {
+  double re = 0.0, im = 0.0;
+  for (size_t n = 0; n < h.size(); ++n) {
+    re += h[n] * std::cos(kPi * omega * n);
+    im -= h[n] * std::sin(kPi * omega * n);
+  }
+  return std::sqrt(re * re + im * im);
+}
+
+// Equiripple AC low-pass from a Function_Filter string ("type=equiripple,
+// fpass=600,fstop=1600,apass=3,astop=60", keys case-insensitive, as parsed by
+// src/params.cpp:1091-1170).  Supported: type equiripple with fpass/fstop,
+// order (or minimum order), apass/astop or wpass/wstop, mode=constrained.
+// h: up to max_taps outputs; n_taps: the length.
+int cm_design_ac_filter(double Fs, const char *function_filter, double *h, int max_taps, int *n_taps) {
+  if (!function_filter || !h || !n_taps || !(Fs > 0.0)) {
+    cm::set_error("cm_design_ac_filter: bad arguments");
+    return -3;
+  }
+  // key=value pairs ([a-zA-Z0-9_]+=[a-zA-Z0-9_.]+), keys and values lower-cased
+  std::map<std::string, std::string> kv;
+  const std::string s(function_filter);
+  auto word = [](char c) { return std::isalnum((unsigned char)c) || c == '_'; };
+  for (size_t i = 0; i < s.size(); ++i) {
+    if (s[i] != '=') continue;
+    size_t ks = i;
+    while (ks > 0 && word(s[ks - 1])) --ks;
+    size_t ve = i + 1;
+    while (ve < s.size() && (word(s[ve]) || s[ve] == '.')) ++ve;
+    if (ks == i || ve == i + 1) continue;
+    std::string k = s.substr(ks, i - ks), v = s.substr(i + 1, ve - i - 1);
+    for (char &c : k) c = (char)std::tolower((unsigned char)c);
+    for (char &c : v) c = (char)std::tolower((unsigned char)c);
+    kv.insert({k, v});
+  }
+  auto num = [&](const char *k, double def) { return kv.count(k) ? std::atof(kv[k].c_str()) : def; };
+  const std::string type = kv.count("type") ? kv["type"] : "equiripple";
+  if (type != "equiripple") {
+    cm::set_error("cm_design_ac_filter: filter type '%s' is not supported (equiripple only)", type.c_str());
+    return -3;
+  }
+  if (kv.count("fc") || !kv.count("fpass") || !kv.count("fstop")) {
+    cm::set_error("cm_design_ac_filter: needs Fpass and Fstop (the Fc form is not supported)");
+    return -3;
+  }
+  const double nyq = (double)(float)(Fs / 2);
+  const double Fpass = (float)num("fpass", 600.0), Fstop = (float)num("fstop", 1600.0);
+  double omega_c = Fpass / nyq;
+  const double pw = (Fstop - Fpass) / nyq;
+  const double max_fstop = Fpass + nyq * 0.3, min_fstop = Fpass + nyq * 0.01;
+  int order = kv.count("order") && kv["order"] != "minimum" ? std::atoi(kv["order"].c_str()) : -1;
+  const bool min_order = order == -1 || (kv.count("minorder") && std::atoi(kv["minorder"].c_str()) == 1);
+  const bool deltas_found = kv.count("astop") || kv.count("apass") ||
+                            (kv.count("mode") && kv["mode"].find("constrained") != std::string::npos);
+  // Apass is 1 dB for the ripple when only Astop is given, the corner
+  // correction then still aims at -3 dB (src/SubModel.cpp:210-213, 273)
+  const double Apass = (float)num("apass", 1.0);
+  const double corner_db = kv.count("apass") ? Apass : 3.0;
+  double dp, ds;
+  if (kv.count("wstop")) {
+    dp = kv.count("wpass") ? (float)num("wpass", 10.0) : 10.0;
+    ds = (float)num("wstop", 100.0);
+  } else if (kv.count("astop")) {
+    dp = std::pow(10.0, -Apass / 20.0);
+    ds = std::pow(10.0, -(double)(float)num("astop", 60.0) / 20.0);
+  } else if (min_order) {
+    cm::set_error("cm_design_ac_filter: minimum order needs Wstop or Astop");
+    return -3;
+  } else {
+    dp = 1.0;
+    ds = 10.0;
+  }
+  if (min_order) order = (int)std::ceil((-10.0 * std::log10(dp * ds) - 13.0) / (2.324 * pw * kPi));
+  double w_stop = 10.0;
+  double w_pass = std::max(0.01, 10.0 * ds / dp);
+  if (w_pass > 10.0 * ds / dp) w_stop *= w_pass / (10.0 * ds / dp);
+  if (kv.count("wstop")) w_stop = (float)num("wstop", 100.0);
+  if (kv.count("wpass")) w_pass = (float)num("wpass", 1.0);
+  if (Fstop > max_fstop || Fstop < min_fstop) {
+    cm::set_error("cm_design_ac_filter: Fstop should be between %g and %g", min_fstop, max_fstop);
+    return -3;
+  }
+  const int taps = order + 1;   // firpm's degree n gives n + 1 coefficients
+  if (order < 2 || taps > max_taps) {
+    cm::set_error("cm_design_ac_filter: %d taps (room for %d)", taps, max_taps);
+    return -3;
+  }
+  std::vector<double> b(taps);
+  auto design = [&](double wc) {
+    if (cm_remez_lowpass(taps, 0.5 * wc, 0.5 * (wc + pw), w_pass, w_stop, 16, b.data(), nullptr, nullptr)) return -3;
+    const double dc = fir_gain(b, 0.0);   // fixDCGain(pmo, 1)
+    for (double &x : b) x /= dc;
+    return 0;
+  };
+  if (design(omega_c)) return -3;
+  if (deltas_found) {
+    // FIRFreqError: last 1/1000-Nyquist step above the -Apass gain before the
+    // response falls below -20 dB, then omega_c <- 2 omega_c - that frequency
+    const double target = std::pow(10.0, -corner_db / 20.0);
+    int j3 = 10;
+    for (int j = 0; j < 1000; ++j) {
+      const double mag = fir_gain(b, j / 1000.0);
+      if (mag > target) j3 = j;
+      if (mag < 0.1) break;
+    }
+    omega_c = std::min(0.99, std::max(0.001, omega_c * 2.0 - j3 / 1000.0));
+    if (design(omega_c)) return -3;
+  }
+  double sum = 0.0;   // setFIRGain(0): unit gain at DC
+  for (double x : b) sum += x;
+  for (int i = 0; i < taps; ++i) h[i] = b[i] / sum;
+  *n_taps = taps;
+  return 0;
+}
+
+}  // extern "C"

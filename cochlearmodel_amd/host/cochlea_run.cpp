@@ -27,7 +27,8 @@
 // JACOBBY_Loops_Fast, JACOBBY_Loops_Slow, Cuda_Outern_Loops, MAX_M1_SP_ERROR,
 // MAX_TOLERANCE, RELATIVE_ERRORS, M1_SP_Fix_Factor, Tolerance_Fix_Factor, SPLRef,
 // Decouple_Filter, Convergence_Tests, Lambda_SAT, Disable_Lambda,
-// AC_Filter_Vector (Filter_Mode 2 vector: [order, isFIR, (a...), b...]).
+// AC_Filter_Vector (Filter_Mode 2 vector: [order, isFIR, (a...), b...]),
+// Function_Filter (equiripple low-pass designed by the library).
 // Run_Stage_Calculation (1: from BM velocity, [totalTimeNodes][256] floats;
 // 2: from lambda, three such planes) with Run_Stage_File_name (raw float32) or
 // Run_Stage_Vector: the stages before it are skipped, as are the convergence
@@ -200,6 +201,38 @@ std::vector<double> read_filter_fixture(const std::string &path) {
   return v;
 }
 
+// The AC filter (SubModel, src/SubModel.cpp:86-131): AC_Filter_Vector
+// (Filter_Mode 2: [order, isFIR, (a...), b...]), else a Function_Filter
+// equiripple design (cm_design_ac_filter), else the committed equiripple
+// design of the default band (data/ac_filter_equiripple_<Fs>.json).
+// Returns true for an FIR (a stays empty).
+bool ac_filter(const ParFile &par, const std::string &exe_dir, float Fs, std::vector<double> &b,
+               std::vector<double> &a) {
+  b.clear();
+  a.clear();
+  if (!par.has("AC_Filter_Vector") && par.has("Function_Filter") && par.scalar<int>("Filter_Mode", 1) == 1) {
+    std::vector<double> h(4096);
+    int n = 0;
+    ok(cm_design_ac_filter(Fs, par.str("Function_Filter").c_str(), h.data(), (int)h.size(), &n),
+       "cm_design_ac_filter");
+    b.assign(h.begin(), h.begin() + n);
+    return true;
+  }
+  const std::vector<double> fm2 = par.has("AC_Filter_Vector")
+                                      ? par.vec<double>("AC_Filter_Vector")
+                                      : read_filter_fixture(exe_dir + "/data/ac_filter_equiripple_" +
+                                                            std::to_string((int)Fs) + ".json");
+  const int order = (int)fm2.at(0);
+  const bool fir = (int)fm2.at(1) == 1;
+  if (fir) {
+    b.assign(fm2.begin() + 2, fm2.begin() + 2 + order);
+  } else {
+    a.assign(fm2.begin() + 2, fm2.begin() + 2 + order);
+    b.assign(fm2.begin() + 2 + order, fm2.begin() + 2 + 2 * order);
+  }
+  return fir;
+}
+
 class BinWriter {
  public:
   explicit BinWriter(const std::string &path) : f_(path, std::ios::binary) {
@@ -355,19 +388,8 @@ int run(const ParFile &par, const std::string &exe_dir) {
   float ihc_ms = 0.f;
   if (!par.scalar<int>("Disable_Lambda", 0) && (allowed & 14)) {
     // AudioGramCreator::setupIHCOHC (src/AudioGramCreator.cpp:603-823)
-    std::vector<double> fm2 = par.has("AC_Filter_Vector")
-                                  ? par.vec<double>("AC_Filter_Vector")
-                                  : read_filter_fixture(exe_dir + "/data/ac_filter_equiripple_" +
-                                                        std::to_string((int)Fs) + ".json");
-    const int order = (int)fm2.at(0);
-    const bool fir = (int)fm2.at(1) == 1;
     std::vector<double> b, a;
-    if (fir) {
-      b.assign(fm2.begin() + 2, fm2.begin() + 2 + order);
-    } else {
-      a.assign(fm2.begin() + 2, fm2.begin() + 2 + order);
-      b.assign(fm2.begin() + 2 + order, fm2.begin() + 2 + 2 * order);
-    }
+    const bool fir = ac_filter(par, exe_dir, Fs, b, a);
     const int ndc = (int)std::lround(Fs * 2e-3f);
     const double dcv = (double)(1.0f / (Fs * 2e-3f));
     std::vector<double> dc(ndc, dcv);
@@ -612,19 +634,8 @@ int run_jnd(const ParFile &par, const std::string &exe_dir) {
   ok(GeneralKernel_Copy_Results_Bytes(failed_blocks.data(), getCudaFailedBlocks(), TB * 4, 0), "failed blocks");
 
   // IHC / lambda over the decoupled intervals
-  std::vector<double> fm2 = par.has("AC_Filter_Vector")
-                                ? par.vec<double>("AC_Filter_Vector")
-                                : read_filter_fixture(exe_dir + "/data/ac_filter_equiripple_" +
-                                                      std::to_string((int)Fs) + ".json");
-  const int order = (int)fm2.at(0);
-  const bool fir = (int)fm2.at(1) == 1;
   std::vector<double> b, a;
-  if (fir) {
-    b.assign(fm2.begin() + 2, fm2.begin() + 2 + order);
-  } else {
-    a.assign(fm2.begin() + 2, fm2.begin() + 2 + order);
-    b.assign(fm2.begin() + 2 + order, fm2.begin() + 2 + 2 * order);
-  }
+  const bool fir = ac_filter(par, exe_dir, Fs, b, a);
   const int ndc = (int)std::lround(Fs * 2e-3f);
   std::vector<double> dc(ndc, (double)(1.0f / (Fs * 2e-3f)));
   std::vector<double> ihcv = par.has("IHC_Vector") ? par.vec<double>("IHC_Vector") : std::vector<double>{8.0};

@@ -42,6 +42,17 @@ def load_ac_filter(fs: float = 44100.0):
         return decode_filter_mode_2(json.load(f)["filter_mode_2"])
 
 
+def design_ac_filter(fs: float, function_filter: str):
+    """Equiripple AC low-pass from a Function_Filter string, designed by the
+    library (SubModel::analyzeACFilter, src/SubModel.cpp:153-360); returns
+    (b, None) like load_ac_filter."""
+    h = np.zeros(4096, np.float64)
+    n = np.zeros(1, np.int32)
+    check(lib().cm_design_ac_filter(float(fs), function_filter.encode(), ptr(h), len(h), ptr(n)),
+          "cm_design_ac_filter")
+    return h[:int(n[0])].copy(), None
+
+
 def dc_filter(fs: float = 44100.0, tdelta: float = TDELTA) -> np.ndarray:
     """Box-car DC filter of setupIHCOHC (src/AudioGramCreator.cpp:806-812)."""
     n = int(round(fs * np.float32(tdelta)))

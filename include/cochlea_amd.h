@@ -221,6 +221,14 @@ int cm_complex_jnd_intervals(int n_freq, int n_levels, int n_noises, int32_t *in
 int cm_complex_jnd(const double *raw, const int32_t *failed, int n_raw, const int32_t *intervals, int n_profiles,
                    int n_levels, int methods, float eps, float *jnd_final, int32_t *warnings, int32_t *conv_warnings);
 
+/* AC filter design (no reference binary counterpart; replaces the firpm-based
+ * equiripple branch of SubModel::analyzeACFilter, src/SubModel.cpp:153-360):
+ * own Parks-McClellan low-pass (bands in cycles per sample) and the
+ * reference's design procedure around it for a Function_Filter string */
+int cm_remez_lowpass(int ntaps, double fpass, double fstop, double wpass, double wstop, int density, double *h,
+                     double *delta, int *iterations);
+int cm_design_ac_filter(double Fs, const char *function_filter, double *h, int max_taps, int *n_taps);
+
 /* ---- input generation -------------------------------------------------- */
 /* replace InputProfilesArrayInitialization / Termination (src/cochlea.cu:2391-2415) */
 int InputProfilesArrayInitialization(int maxJNDIntervals, int wn_length, int signal_length, int signal_mode,

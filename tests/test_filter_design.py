@@ -1,0 +1,85 @@
+"""AC filter design of the library (host code): the Parks-McClellan low-pass
+against scipy.signal.remez on the same grid, its equiripple alternation, and
+the reference's equiripple procedure around it (minimum order, unit DC gain,
+error cases of SubModel::analyzeACFilter)."""
+import numpy as np
+import pytest
+
+CASES = [(57, 600 / 44100, 1600 / 44100, 0.014125, 10.0), (30, 0.1, 0.15, 1.0, 10.0), (101, 0.2, 0.25, 1.0, 10.0),
+         (76, 600 / 44100, 1600 / 44100, 0.01, 10.0), (25, 0.1, 0.2, 1.0, 1.0), (200, 0.05, 0.06, 1.0, 5.0),
+         (64, 0.1, 0.14, 3.0, 1.0), (151, 0.02, 0.05, 0.2, 10.0)]
+
+
+def _remez(n, fp, fs, wp, ws):
+    from cochlearmodel_amd import _lib
+    h = np.zeros(n)
+    d = np.zeros(1)
+    it = np.zeros(1, np.int32)
+    _lib.check(_lib.lib().cm_remez_lowpass(n, fp, fs, wp, ws, 16, _lib.ptr(h), _lib.ptr(d), _lib.ptr(it)))
+    return h, float(d[0]), int(it[0])
+
+
+@pytest.mark.parametrize("n,fp,fs,wp,ws", CASES)
+def test_remez_matches_scipy(n, fp, fs, wp, ws):
+    signal = pytest.importorskip("scipy.signal")
+    h, _, _ = _remez(n, fp, fs, wp, ws)
+    ref = signal.remez(n, [0, fp, fs, 0.5], [1, 0], weight=[wp, ws], fs=1.0, grid_density=16, maxiter=200)
+    assert np.abs(h - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max())
+    np.testing.assert_array_equal(h, h[::-1])
+
+
+@pytest.mark.parametrize("n,fp,fs,wp,ws", CASES[:4])
+def test_remez_equiripple_alternation(n, fp, fs, wp, ws):
+    """Chebyshev alternation on the design grid: the weighted error of the real
+    amplitude reaches +-delta, alternating in sign, at (number of cosine basis
+    functions + 1) frequencies, and between grid points exceeds delta by at
+    most a grid-discretisation margin."""
+    h, delta, _ = _remez(n, fp, fs, wp, ws)
+    n_basis = n // 2 if n % 2 == 0 else n // 2 + 1
+    step = 0.5 / (16 * n_basis)
+    signs = []
+    for lo, hi, d, w in ((0.0, fp, 1.0, wp), (fs, 0.5, 0.0, ws)):
+        f = np.concatenate([np.arange(lo, hi, step / 8), [hi]])
+        A = np.real(np.exp(-2j * np.pi * np.outer(f, np.arange(n) - (n - 1) / 2)) @ h)
+        err = w * (d - A)
+        assert np.abs(err).max() <= delta * 1.02
+        # local extrema of this band that reach delta
+        ext = [i for i in range(len(f)) if abs(err[i]) >= delta * (1 - 1e-6) and
+               (i == 0 or abs(err[i]) >= abs(err[i - 1])) and (i == len(f) - 1 or abs(err[i]) >= abs(err[i + 1]))]
+        signs.extend(np.sign(err[ext]))
+    alternations = 1 + np.count_nonzero(np.diff(signs) != 0)
+    assert alternations >= n_basis + 1
+
+
+def _design(fs, spec):
+    from cochlearmodel_amd import _lib
+    h = np.zeros(4096)
+    n = np.zeros(1, np.int32)
+    rc = _lib.lib().cm_design_ac_filter(fs, spec.encode(), _lib.ptr(h), len(h), _lib.ptr(n))
+    return rc, h[:int(n[0])]
+
+
+def test_ac_filter_design_procedure():
+    """Default band at 44.1 kHz: the minimum order of Oppenheim & Schafer 7.104
+    with delta_pass = 10^(-3/20), delta_stop = 10^(-60/20) and transition
+    1000/22050 is ceil(18.4996 / 0.331108) = 56, i.e. 57 coefficients; the design
+    has unit DC gain and is a low-pass."""
+    rc, h = _design(44100.0, "type=equiripple,fpass=600,fstop=1600,apass=3,astop=60")
+    assert rc == 0 and len(h) == 57
+    assert abs(h.sum() - 1.0) < 1e-12
+    np.testing.assert_array_equal(h, h[::-1])
+    f = np.array([0.0, 300.0, 3000.0, 10000.0]) / 44100
+    H = np.abs(np.exp(-2j * np.pi * np.outer(f, np.arange(len(h)))) @ h)
+    assert abs(H[0] - 1) < 1e-12 and H[1] > 0.7 and H[2] < 0.02 and H[3] < 0.02
+    # keys are case-insensitive, and the design is deterministic
+    rc2, h2 = _design(44100.0, "Type=EquiRipple,Fpass=600,Fstop=1600,Apass=3,Astop=60")
+    assert rc2 == 0
+    np.testing.assert_array_equal(h, h2)
+
+
+@pytest.mark.parametrize("spec", ["type=butterworth,fpass=600,fstop=1600", "type=equiripple,fc=800,astop=60",
+                                  "type=equiripple,fpass=600,fstop=9000,astop=60", "type=equiripple,fpass=600"])
+def test_ac_filter_design_refuses(spec):
+    from cochlearmodel_amd import _lib
+    rc, _ = _design(44100.0, spec)
+    assert rc == -3 and _lib.lib().cm_last_error()

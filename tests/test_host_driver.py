@@ -149,24 +149,27 @@ def test_resume_needs_stage_data(tmp_path):
 
 
 @pytest.mark.gpu
-def test_resume_from_bm_velocity(tmp_path, oracle):
+@pytest.mark.parametrize("function_filter", [None, "type=equiripple,fpass=600,fstop=1600,apass=3,astop=60"])
+def test_resume_from_bm_velocity(tmp_path, oracle, function_filter):
     """Run_Stage_Calculation = 1: the BM stage is skipped, the given velocity is
-    uploaded, and the lambda planes equal the oracle's IHC/lambda of it."""
+    uploaded, and the lambda planes equal the oracle's IHC/lambda of it (with
+    the committed AC filter, or one designed from a Function_Filter string)."""
     from cochlearmodel_amd import solver, stages
     lay = solver.voice_layout(FS, 5, 0.02, 0.005)
     TB, ov, tot = lay.time_blocks, lay.overlap_nodes, lay.total_nodes
-    print(f"case: cochlea_run resumed from BM velocity, {tot} rows", flush=True)
+    print(f"case: cochlea_run resumed from BM velocity, {tot} rows, Function_Filter {function_filter}", flush=True)
     rng = np.random.default_rng(11)
     t = np.arange(tot)[:, None] / FS
     vel = (1e-4 * np.sin(2 * np.pi * 900.0 * t + np.linspace(0, 3, 256)[None, :]) *
            np.exp(-np.linspace(0, 4, 256))[None, :] + 1e-6 * rng.standard_normal((tot, 256))).astype(np.float32)
     f = tmp_path / "velocity.f32"
     vel.tofile(f)
-    par, out = _voice_par(tmp_path, f"Run_Stage_Calculation = 1\nRun_Stage_File_name = {f}\n")
+    extra = f"Function_Filter = {function_filter}\n" if function_filter else ""
+    par, out = _voice_par(tmp_path, f"Run_Stage_Calculation = 1\nRun_Stage_File_name = {f}\n" + extra)
     r = subprocess.run([EXE, str(par)], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
     got = np.fromfile(out, np.float32)
-    b, _ = stages.load_ac_filter(FS)
+    b, _ = stages.design_ac_filter(FS, function_filter) if function_filter else stages.load_ac_filter(FS)
     lo = oracle.ihc_lambda(vel, tot, lay.block_nodes * TB, b, stages.dc_filter(FS), stages.ihc_damage(8.0),
                            stages.aihc_table(), np.array(stages.SPONT_DEFAULT, np.float32), threads=8)
     expect = np.concatenate([vel[ov:tot].ravel()] + [lo[j, ov:tot].ravel() for j in range(3)])
