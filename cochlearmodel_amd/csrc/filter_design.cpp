@@ -101,6 +101,51 @@ struct Interp {
   }
 };
 
+// sin(x)/x, 1 near 0 (|x| < 1e-5), as the Iowa Hills routines define it
+double sinc(double x) { return (x > -1.0e-5 && x < 1.0e-5) ? 1.0 : std::sin(x) / x; }
+
+// the Kaiser window's I0 as the reference evaluates it: the power series
+// 1 + sum_{i=1..9} ((x/2)^i / i!)^2, truncated after nine terms
+// (src/IowaHillsFilters/FFTCode.cpp:771-783)
+double bessel_i0_9(double x) {
+  double sum = 0.0, term = 1.0;
+  for (int i = 1; i < 10; ++i) {
+    term *= (x / 2.0) / i;
+    sum += term * term;
+  }
+  return 1.0 + sum;
+}
+
+// rectangular-window low-pass: omega_c sinc(omega_c pi (j - (n-1)/2))
+// (RectWinFIR, firLPF, src/IowaHillsFilters/FIRFilterCode.cpp:29-42)
+void rect_lowpass(std::vector<double> &h, double omega_c) {
+  const int n = (int)h.size();
+  for (int j = 0; j < n; ++j) h[j] = omega_c * sinc(omega_c * (j - (n - 1) / 2.0) * kPi);
+}
+
+enum class Win { None, Kaiser, KaiserBessel, Sinc, Hanning, Hamming, Blackman, Flattop };
+
+// FIRFilterWindow (src/IowaHillsFilters/FIRFilterCode.cpp:108-150): only the
+// Kaiser and Sinc (Lanczos) windows are applied there; the other names leave
+// the coefficients unwindowed.  beta is clamped to [0, 10].
+void apply_window(std::vector<double> &h, Win w, double beta) {
+  if (w != Win::Kaiser && w != Win::Sinc) return;
+  beta = std::min(10.0, std::max(0.0, beta));
+  const int n = (int)h.size();
+  const double dn = n + 1.0;
+  std::vector<double> c(n);
+  for (int j = 0; j < n; ++j) {
+    if (w == Win::Kaiser) {
+      const double t = ((double)(2 * j + 2) - dn) / dn;
+      c[j] = bessel_i0_9(beta * std::sqrt(1.0 - t * t)) / bessel_i0_9(beta);
+    } else {
+      c[j] = std::pow(sinc((double)(2 * j + 1 - n) / dn * kPi), beta);
+    }
+  }
+  for (int j = 0; j < n / 2; ++j) c[n - j - 1] = c[j];
+  for (int j = 0; j < n; ++j) h[j] *= c[j];
+}
+
 }  // namespace
 
 extern "C" {
@@ -215,6 +260,92 @@ static double fir_gain(const std::vector<double> &h, double omega) {
   return std::sqrt(re * re + im * im);
 }
 
+// FIRFreqError (src/firpm/pm.cpp:742-793), low-pass: the last 1/1000-Nyquist
+// step whose gain exceeds -corner_db before the response falls below -20 dB;
+// returns the corrected cut-off 2 omega_c - that frequency, clamped.
+static double corner_correction(const std::vector<double> &h, double omega_c, double corner_db) {
+  const double target = std::pow(10.0, -corner_db / 20.0);
+  int j3 = 10;
+  for (int j = 0; j < 1000; ++j) {
+    const double mag = fir_gain(h, j / 1000.0);
+    if (mag > target) j3 = j;
+    if (mag < 0.1) break;
+  }
+  return std::min(0.99, std::max(0.001, omega_c * 2.0 - j3 / 1000.0));
+}
+
+// Window branch of SubModel::analyzeACFilter (src/SubModel.cpp:366-457).
+static int design_window(double Fs, std::map<std::string, std::string> &kv, double *h, int max_taps, int *n_taps) {
+  if (!kv.count("windowtype")) {
+    cm::set_error("cm_design_ac_filter: FIR filter with Window must have window type");
+    return -3;
+  }
+  const std::string wn = kv["windowtype"];
+  const Win w = wn == "kaiser" ? Win::Kaiser : wn == "kaiserbessel" ? Win::KaiserBessel : wn == "sinc" ? Win::Sinc
+              : wn == "hanning" ? Win::Hanning : wn == "hamming" ? Win::Hamming : wn == "blackmann" ? Win::Blackman
+              : wn == "flattop" ? Win::Flattop : Win::None;
+  auto num = [&](const char *k, double def) { return kv.count(k) ? std::atof(kv[k].c_str()) : def; };
+  const double nyq = (double)(float)(Fs / 2);
+  int taps = kv.count("order") ? std::atoi(kv["order"].c_str()) : -1;
+  const double Apass = kv.count("apass") ? (double)(float)num("apass", 3.0) : 3.0;
+  const double Astop = kv.count("astop") ? (double)(float)num("astop", 0.0) : 0.0;
+  double omega_c, beta = 0.0;
+  if (kv.count("fc")) {
+    if (!kv.count("order")) {
+      cm::set_error("cm_design_ac_filter: FIR filter with Fc must get specific Order");
+      return -3;
+    }
+    omega_c = (double)(float)num("fc", 800.0) / nyq;
+  } else {
+    if (!kv.count("fpass") || !kv.count("fstop")) {
+      cm::set_error("cm_design_ac_filter: FIR filter without Fc must have Fpass,Fstop");
+      return -3;
+    }
+    const float fp = (float)num("fpass", 600.0), fs = (float)num("fstop", 1600.0);
+    omega_c = (double)(fp + fs) / (2 * nyq);
+    const double tw = (fs - fp) / nyq;
+    switch (w) {
+      case Win::None: taps = (int)(4 * kPi / tw); break;
+      case Win::Hamming:
+      case Win::Hanning: taps = (int)(8 * kPi / tw); break;
+      case Win::Blackman: taps = (int)(12 * kPi / tw); break;
+      case Win::Kaiser: {
+        if (Astop == 0 && !kv.count("apass")) {
+          cm::set_error("cm_design_ac_filter: Kaiser window must have Astop or Apass");
+          return -3;
+        }
+        const double twp = tw * kPi, a = std::max(Apass, Astop);
+        taps = (int)std::ceil(a > 21 ? (a - 7.95) / (2.285 * twp) : 5.79 / twp);
+        if (kv.count("beta")) beta = num("beta", 0.0);
+        else if (a > 50) beta = 0.1102 * (a - 8.7);
+        // the reference raises (A - 21) to the 4th power here (Kaiser's rule has 0.4)
+        else if (a > 21) beta = 0.5842 * std::pow(a - 21, 4) + 0.07886 * (a - 21);
+        break;
+      }
+      default: break;   // Sinc, Flattop, KaiserBessel: the order must be given
+    }
+    if (kv.count("order")) taps = std::atoi(kv["order"].c_str());
+  }
+  if (taps < 3 || taps > max_taps) {
+    cm::set_error("cm_design_ac_filter: window design of %d taps (give Order; room for %d)", taps, max_taps);
+    return -3;
+  }
+  std::vector<double> b(taps);
+  auto design = [&](double wc) {
+    rect_lowpass(b, wc);
+    apply_window(b, w, beta);
+    const double dc = fir_gain(b, 0.0);   // fixDCGain
+    for (double &x : b) x /= dc;
+  };
+  design(omega_c);
+  design(corner_correction(b, omega_c, 3.0));
+  double sum = 0.0;
+  for (double x : b) sum += x;
+  for (int i = 0; i < taps; ++i) h[i] = b[i] / sum;
+  *n_taps = taps;
+  return 0;
+}
+
 // Equiripple AC low-pass from a Function_Filter string ("type=equiripple,
 // fpass=600,fstop=1600,apass=3,astop=60", keys case-insensitive, as parsed by
 // src/params.cpp:1091-1170).  Supported: type equiripple with fpass/fstop,
@@ -243,8 +374,9 @@ int cm_design_ac_filter(double Fs, const char *function_filter, double *h, int m
   }
   auto num = [&](const char *k, double def) { return kv.count(k) ? std::atof(kv[k].c_str()) : def; };
   const std::string type = kv.count("type") ? kv["type"] : "equiripple";
+  if (type == "window") return design_window(Fs, kv, h, max_taps, n_taps);
   if (type != "equiripple") {
-    cm::set_error("cm_design_ac_filter: filter type '%s' is not supported (equiripple only)", type.c_str());
+    cm::set_error("cm_design_ac_filter: filter type '%s' is not supported (equiripple or window)", type.c_str());
     return -3;
   }
   if (kv.count("fc") || !kv.count("fpass") || !kv.count("fstop")) {
@@ -302,16 +434,7 @@ int cm_design_ac_filter(double Fs, const char *function_filter, double *h, int m
   };
   if (design(omega_c)) return -3;
   if (deltas_found) {
-    // FIRFreqError: last 1/1000-Nyquist step above the -Apass gain before the
-    // response falls below -20 dB, then omega_c <- 2 omega_c - that frequency
-    const double target = std::pow(10.0, -corner_db / 20.0);
-    int j3 = 10;
-    for (int j = 0; j < 1000; ++j) {
-      const double mag = fir_gain(b, j / 1000.0);
-      if (mag > target) j3 = j;
-      if (mag < 0.1) break;
-    }
-    omega_c = std::min(0.99, std::max(0.001, omega_c * 2.0 - j3 / 1000.0));
+    omega_c = corner_correction(b, omega_c, corner_db);
     if (design(omega_c)) return -3;
   }
   double sum = 0.0;   // setFIRGain(0): unit gain at DC
