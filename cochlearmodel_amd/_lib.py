@@ -110,7 +110,7 @@ _SIGS = {
     "cm_jnd_final": (C.c_int, [vp, vp, C.c_int, C.c_int, C.c_int, vp, C.c_int, vp, f64, f64, C.c_int, C.c_int, vp,
                                vp, vp, vp]),
     "cm_remez_lowpass": (C.c_int, [C.c_int, f64, f64, f64, f64, C.c_int, vp, vp, vp]),
-    "cm_design_ac_filter": (C.c_int, [f64, C.c_char_p, vp, C.c_int, vp]),
+    "cm_design_ac_filter": (C.c_int, [f64, C.c_char_p, vp, vp, C.c_int, vp, vp]),
     "cm_jnd_failed_signals": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp, C.c_int, vp, C.c_int, vp]),
     "InputProfilesArrayInitialization": (C.c_int, [C.c_int] * 5),
     "InputProfilesArrayTermination": (C.c_int, []),

@@ -346,16 +346,98 @@ static int design_window(double Fs, std::map<std::string, std::string> &kv, doub
   return 0;
 }
 
-// Equiripple AC low-pass from a Function_Filter string ("type=equiripple,
-// fpass=600,fstop=1600,apass=3,astop=60", keys case-insensitive, as parsed by
-// src/params.cpp:1091-1170).  Supported: type equiripple with fpass/fstop,
-// order (or minimum order), apass/astop or wpass/wstop, mode=constrained.
-// h: up to max_taps outputs; n_taps: the length.
-int cm_design_ac_filter(double Fs, const char *function_filter, double *h, int max_taps, int *n_taps) {
-  if (!function_filter || !h || !n_taps || !(Fs > 0.0)) {
+// polynomial product (coefficients in ascending powers of z^-1)
+static std::vector<double> poly_mul(const std::vector<double> &p, const std::vector<double> &q) {
+  std::vector<double> r(p.size() + q.size() - 1, 0.0);
+  for (size_t i = 0; i < p.size(); ++i)
+    for (size_t j = 0; j < q.size(); ++j) r[i + j] += p[i] * q[j];
+  return r;
+}
+
+// Butterworth branch of SubModel::analyzeACFilter (src/SubModel.cpp:458-503):
+// order ceil(log(sqrt(Gs^2 - 1) / eps) / log(ws / wp)) with eps^2 = Gp^2 - 1,
+// cut-off wp eps^(-1/N) (pass-band mode) or ws (eps / sqrt(Gs^2 - 1))^(1/N)
+// (stop-band mode); the analog prototype's poles exp(j pi (2k + N + 1) / 2N)
+// taken most negative real part first, one 1st- or 2nd-order factor each,
+// mapped by the bilinear transform with s = T (1 - z^-1) / (1 + z^-1),
+// T = 2 tan(pi omega_c / 2), each factor with unit DC gain
+// (CalcIIRFilterCoeff, src/IowaHillsFilters/IIRFilterCode.cpp:32-140), and the
+// factors multiplied as 5-coefficient sections (HFunction products,
+// src/HFunction.cpp:91-126), so N poles give 4 ceil(N/2) + 1 coefficients.
+static int design_butterworth(double Fs, std::map<std::string, std::string> &kv, double *b_out, double *a_out,
+                              int max_len, int *n_out) {
+  auto num = [&](const char *k, double def) { return kv.count(k) ? std::atof(kv[k].c_str()) : def; };
+  const double nyq = (double)(float)(Fs / 2);
+  const double Apass = (float)num("apass", 3.0), Astop = (float)num("astop", 60.0);
+  const double wp = (double)(float)num("fpass", 600.0) / nyq, ws = (double)(float)num("fstop", 1600.0) / nyq;
+  const double h0 = std::pow(10.0, Apass / 20.0), gs = std::pow(10.0, Astop / 20.0);
+  const double eps = std::sqrt(h0 * h0 - 1.0);
+  if (!(eps > 0.0) || !(gs > 1.0) || !(ws > wp) || !(wp > 0.0)) {
+    cm::set_error("cm_design_ac_filter: Butterworth needs Apass > 0, Astop > 0 and Fstop > Fpass > 0");
+    return -3;
+  }
+  const int order = (int)std::ceil(std::log(std::sqrt(gs * gs - 1.0) / eps) / std::log(ws / wp));
+  const bool stopband = kv.count("mode") && kv["mode"].find("stopband") != std::string::npos;
+  const double wc = stopband ? ws * std::pow(eps / std::sqrt(gs * gs - 1.0), 1.0 / order)
+                             : wp * std::pow(eps, -1.0 / order);
+  const int sections = (order + 1) / 2;
+  if (order < 1 || order > 40 || 4 * sections + 1 > max_len) {
+    cm::set_error("cm_design_ac_filter: Butterworth order %d out of range", order);
+    return -3;
+  }
+  // prototype poles, most negative real part first
+  std::vector<std::pair<double, double>> poles;
+  for (int j = 0; j < order / 2; ++j) {
+    const double th = kPi * (double)(2 * j + order + 1) / (double)(2 * order);
+    poles.push_back({std::cos(th), std::sin(th)});
+  }
+  if (order % 2) poles.push_back({-1.0, 0.0});
+  std::stable_sort(poles.begin(), poles.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
+  const double T = 2.0 * std::tan(wc * kPi / 2.0);
+  std::vector<double> B{1.0}, A{1.0};
+  for (const auto &p : poles) {
+    std::vector<double> nb(5, 0.0), na(5, 0.0);
+    na[0] = 1.0;
+    if (p.second == 0.0) {   // real pole: s + |re|
+      const double Bc = 1.0, Cc = -p.first, arg = 2.0 * Bc + Cc * T;
+      na[1] = (-2.0 * Bc + Cc * T) / arg;
+      nb[0] = Cc * T / arg;
+      nb[1] = Cc * T / arg;
+    } else {                 // pair: s^2 - 2 re s + |p|^2
+      const double Ac = 1.0, Bc = -2.0 * p.first, Cc = p.first * p.first + p.second * p.second;
+      const double arg = 4.0 * Ac + 2.0 * Bc * T + Cc * T * T;
+      na[1] = (2.0 * Cc * T * T - 8.0 * Ac) / arg;
+      na[2] = (4.0 * Ac - 2.0 * Bc * T + Cc * T * T) / arg;
+      nb[0] = Cc * T * T / arg;
+      nb[1] = 2.0 * Cc * T * T / arg;
+      nb[2] = Cc * T * T / arg;
+    }
+    B = B.size() == 1 ? nb : poly_mul(B, nb);
+    A = A.size() == 1 ? na : poly_mul(A, na);
+  }
+  for (size_t i = 0; i < B.size(); ++i) {
+    b_out[i] = B[i];
+    a_out[i] = A[i];
+  }
+  *n_out = (int)B.size();
+  return 0;
+}
+
+// AC low-pass from a Function_Filter string ("type=equiripple,fpass=600,
+// fstop=1600,apass=3,astop=60", keys case-insensitive, as parsed by
+// src/params.cpp:1091-1170).  Types: equiripple (fpass/fstop, order or minimum
+// order, apass/astop or wpass/wstop, mode=constrained), window (windowtype,
+// fc+order or fpass/fstop, astop/apass, beta), butterworth (fpass, fstop,
+// apass, astop, mode=stopband).  h: numerator, a: denominator (Butterworth
+// only; may be NULL otherwise), up to max_taps each; n_taps: the length;
+// is_iir: 1 for the Butterworth IIR.
+int cm_design_ac_filter(double Fs, const char *function_filter, double *h, double *a, int max_taps, int *n_taps,
+                        int *is_iir) {
+  if (!function_filter || !h || !n_taps || !is_iir || !(Fs > 0.0)) {
     cm::set_error("cm_design_ac_filter: bad arguments");
     return -3;
   }
+  *is_iir = 0;
   // key=value pairs ([a-zA-Z0-9_]+=[a-zA-Z0-9_.]+), keys and values lower-cased
   std::map<std::string, std::string> kv;
   const std::string s(function_filter);
@@ -375,8 +457,16 @@ int cm_design_ac_filter(double Fs, const char *function_filter, double *h, int m
   auto num = [&](const char *k, double def) { return kv.count(k) ? std::atof(kv[k].c_str()) : def; };
   const std::string type = kv.count("type") ? kv["type"] : "equiripple";
   if (type == "window") return design_window(Fs, kv, h, max_taps, n_taps);
+  if (type == "butterworth") {
+    if (!a) {
+      cm::set_error("cm_design_ac_filter: a Butterworth design needs the denominator array");
+      return -3;
+    }
+    *is_iir = 1;
+    return design_butterworth(Fs, kv, h, a, max_taps, n_taps);
+  }
   if (type != "equiripple") {
-    cm::set_error("cm_design_ac_filter: filter type '%s' is not supported (equiripple or window)", type.c_str());
+    cm::set_error("cm_design_ac_filter: filter type '%s' is not supported", type.c_str());
     return -3;
   }
   if (kv.count("fc") || !kv.count("fpass") || !kv.count("fstop")) {

@@ -28,7 +28,8 @@
 // MAX_TOLERANCE, RELATIVE_ERRORS, M1_SP_Fix_Factor, Tolerance_Fix_Factor, SPLRef,
 // Decouple_Filter, Convergence_Tests, Lambda_SAT, Disable_Lambda,
 // AC_Filter_Vector (Filter_Mode 2 vector: [order, isFIR, (a...), b...]),
-// Function_Filter (equiripple low-pass designed by the library).
+// Function_Filter (equiripple, window or Butterworth low-pass designed by the
+// library).
 // Run_Stage_Calculation (1: from BM velocity, [totalTimeNodes][256] floats;
 // 2: from lambda, three such planes) with Run_Stage_File_name (raw float32) or
 // Run_Stage_Vector: the stages before it are skipped, as are the convergence
@@ -211,12 +212,13 @@ bool ac_filter(const ParFile &par, const std::string &exe_dir, float Fs, std::ve
   b.clear();
   a.clear();
   if (!par.has("AC_Filter_Vector") && par.has("Function_Filter") && par.scalar<int>("Filter_Mode", 1) == 1) {
-    std::vector<double> h(4096);
-    int n = 0;
-    ok(cm_design_ac_filter(Fs, par.str("Function_Filter").c_str(), h.data(), (int)h.size(), &n),
+    std::vector<double> h(4096), den(4096);
+    int n = 0, iir = 0;
+    ok(cm_design_ac_filter(Fs, par.str("Function_Filter").c_str(), h.data(), den.data(), (int)h.size(), &n, &iir),
        "cm_design_ac_filter");
     b.assign(h.begin(), h.begin() + n);
-    return true;
+    if (iir) a.assign(den.begin(), den.begin() + n);
+    return !iir;
   }
   const std::vector<double> fm2 = par.has("AC_Filter_Vector")
                                       ? par.vec<double>("AC_Filter_Vector")

@@ -43,14 +43,17 @@ def load_ac_filter(fs: float = 44100.0):
 
 
 def design_ac_filter(fs: float, function_filter: str):
-    """Equiripple AC low-pass from a Function_Filter string, designed by the
-    library (SubModel::analyzeACFilter, src/SubModel.cpp:153-360); returns
-    (b, None) like load_ac_filter."""
+    """AC low-pass from a Function_Filter string, designed by the library
+    (SubModel::analyzeACFilter, src/SubModel.cpp:153-503); returns (b, a) like
+    load_ac_filter, a = None for the FIR designs."""
     h = np.zeros(4096, np.float64)
+    a = np.zeros(4096, np.float64)
     n = np.zeros(1, np.int32)
-    check(lib().cm_design_ac_filter(float(fs), function_filter.encode(), ptr(h), len(h), ptr(n)),
+    iir = np.zeros(1, np.int32)
+    check(lib().cm_design_ac_filter(float(fs), function_filter.encode(), ptr(h), ptr(a), len(h), ptr(n), ptr(iir)),
           "cm_design_ac_filter")
-    return h[:int(n[0])].copy(), None
+    k = int(n[0])
+    return h[:k].copy(), (a[:k].copy() if iir[0] else None)
 
 
 def dc_filter(fs: float = 44100.0, tdelta: float = TDELTA) -> np.ndarray:

@@ -227,7 +227,8 @@ int cm_complex_jnd(const double *raw, const int32_t *failed, int n_raw, const in
  * reference's design procedure around it for a Function_Filter string */
 int cm_remez_lowpass(int ntaps, double fpass, double fstop, double wpass, double wstop, int density, double *h,
                      double *delta, int *iterations);
-int cm_design_ac_filter(double Fs, const char *function_filter, double *h, int max_taps, int *n_taps);
+int cm_design_ac_filter(double Fs, const char *function_filter, double *h, double *a, int max_taps, int *n_taps,
+                        int *is_iir);
 
 /* ---- input generation -------------------------------------------------- */
 /* replace InputProfilesArrayInitialization / Termination (src/cochlea.cu:2391-2415) */

@@ -54,8 +54,13 @@ def test_remez_equiripple_alternation(n, fp, fs, wp, ws):
 def _design(fs, spec):
     from cochlearmodel_amd import _lib
     h = np.zeros(4096)
+    a = np.zeros(4096)
     n = np.zeros(1, np.int32)
-    rc = _lib.lib().cm_design_ac_filter(fs, spec.encode(), _lib.ptr(h), len(h), _lib.ptr(n))
+    iir = np.zeros(1, np.int32)
+    rc = _lib.lib().cm_design_ac_filter(fs, spec.encode(), _lib.ptr(h), _lib.ptr(a), len(h), _lib.ptr(n),
+                                        _lib.ptr(iir))
+    if iir[0]:
+        return rc, (h[:int(n[0])], a[:int(n[0])])
     return rc, h[:int(n[0])]
 
 
@@ -77,7 +82,7 @@ def test_ac_filter_design_procedure():
     np.testing.assert_array_equal(h, h2)
 
 
-@pytest.mark.parametrize("spec", ["type=butterworth,fpass=600,fstop=1600", "type=equiripple,fc=800,astop=60",
+@pytest.mark.parametrize("spec", ["type=elliptic,fpass=600,fstop=1600", "type=equiripple,fc=800,astop=60",
                                   "type=equiripple,fpass=600,fstop=9000,astop=60", "type=equiripple,fpass=600"])
 def test_ac_filter_design_refuses(spec):
     from cochlearmodel_amd import _lib
@@ -126,3 +131,32 @@ def test_window_design_unwindowed_names_and_fc_form():
 def test_window_design_refuses(spec):
     rc, _ = _design(44100.0, spec)
     assert rc == -3
+
+
+def test_butterworth_design():
+    """Butterworth branch: for Apass 3 dB, Astop 60 dB, 600/1600 Hz the order
+    ceil(ln(sqrt(10^6 - 1) / eps) / ln(1600/600)) with eps^2 = 10^0.3 - 1 is
+    ceil(7.045) = 8, i.e. 4 sections and 17 coefficients (5-term section
+    products); the response is the bilinear image of the analog Butterworth,
+    |H|^2 = 1 / (1 + (tan(pi f / Fs) / tan(pi wc / 2))^16), wc = wp eps^(-1/8),
+    and equals scipy.signal.butter of the same order and cut-off."""
+    signal = pytest.importorskip("scipy.signal")
+    rc, (b, a) = _design(44100.0, "type=butterworth,fpass=600,fstop=1600,apass=3,astop=60")
+    assert rc == 0 and len(b) == 17 and len(a) == 17
+    assert np.all(b[9:] == 0) and np.all(a[9:] == 0) and a[0] == 1.0
+    eps = np.sqrt(10 ** 0.3 - 1)
+    wc = 600 / 22050 * eps ** (-1 / 8)
+    bs, as_ = signal.butter(8, wc)
+    np.testing.assert_allclose(b[:9], bs, rtol=1e-9, atol=1e-20)
+    np.testing.assert_allclose(a[:9], as_, rtol=1e-9, atol=1e-12)
+    f = np.linspace(10.0, 5000.0, 400)
+    sos = signal.tf2sos(b[:9], a[:9])
+    _, H = signal.sosfreqz(sos, worN=f, fs=44100.0)
+    ref = 1 / np.sqrt(1 + (np.tan(np.pi * f / 44100) / np.tan(np.pi * wc / 2)) ** 16)
+    np.testing.assert_allclose(np.abs(H), ref, atol=1e-5)   # direct-form coefficients of order 8 carry ~5e-6
+    # stop-band mode: the cut-off comes from Fstop and Astop instead
+    rc2, (b2, a2) = _design(44100.0, "type=butterworth,fpass=600,fstop=1600,apass=3,astop=60,mode=stopband")
+    wc2 = 1600 / 22050 * (eps / np.sqrt(1e6 - 1)) ** (1 / 8)
+    bs2, as2 = signal.butter(8, wc2)
+    assert rc2 == 0
+    np.testing.assert_allclose(a2[:9], as2, rtol=1e-9, atol=1e-12)

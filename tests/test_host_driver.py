@@ -149,7 +149,9 @@ def test_resume_needs_stage_data(tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("function_filter", [None, "type=equiripple,fpass=600,fstop=1600,apass=3,astop=60"])
+@pytest.mark.parametrize("function_filter", [None, "type=equiripple,fpass=600,fstop=1600,apass=3,astop=60",
+                                             "type=window,windowtype=kaiser,fpass=600,fstop=1600,astop=60",
+                                             "type=butterworth,fpass=600,fstop=2400,apass=3,astop=30"])
 def test_resume_from_bm_velocity(tmp_path, oracle, function_filter):
     """Run_Stage_Calculation = 1: the BM stage is skipped, the given velocity is
     uploaded, and the lambda planes equal the oracle's IHC/lambda of it (with
@@ -169,12 +171,19 @@ def test_resume_from_bm_velocity(tmp_path, oracle, function_filter):
     r = subprocess.run([EXE, str(par)], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
     got = np.fromfile(out, np.float32)
-    b, _ = stages.design_ac_filter(FS, function_filter) if function_filter else stages.load_ac_filter(FS)
+    b, a = stages.design_ac_filter(FS, function_filter) if function_filter else stages.load_ac_filter(FS)
     lo = oracle.ihc_lambda(vel, tot, lay.block_nodes * TB, b, stages.dc_filter(FS), stages.ihc_damage(8.0),
-                           stages.aihc_table(), np.array(stages.SPONT_DEFAULT, np.float32), threads=8)
+                           stages.aihc_table(), np.array(stages.SPONT_DEFAULT, np.float32), threads=8, ac_a=a)
     expect = np.concatenate([vel[ov:tot].ravel()] + [lo[j, ov:tot].ravel() for j in range(3)])
     assert got.size == expect.size + 2
-    np.testing.assert_array_equal(got[:-2], expect)
+    nv = (tot - ov) * 256
+    np.testing.assert_array_equal(got[:nv], expect[:nv])
+    if a is None:
+        np.testing.assert_array_equal(got[nv:-2], expect[nv:])
+    else:   # IIR: serial float accumulation, compared as the IIR stage test does
+        rel = np.linalg.norm(got[nv:-2] - expect[nv:]) / np.linalg.norm(expect[nv:])
+        print(f"IIR lambda rel L2 {rel:.2e}", flush=True)
+        assert rel <= 1e-4
     assert got[-2] == 0.0                      # no BM kernel ran
 
 
