@@ -727,8 +727,23 @@ int main(int argc, char **argv) {
       for (const auto &kv : par.map()) std::printf("%s = %s\n", kv.first.c_str(), kv.second.c_str());
       return 0;
     }
+    if (argc == 3 && std::string(argv[1]) == "--profiles") {
+      // the JND interval layout of a profile run, one line per interval
+      ParFile par(argv[2]);
+      const std::vector<float> freqs = par.vec<float>("testedFrequencies"), levels = par.vec<float>("testedPowerLevels");
+      const std::vector<float> noises = par.has("testedNoises") ? par.vec<float>("testedNoises")
+                                                                : std::vector<float>{1111.f};
+      const JndLayout lay = jnd_layout(freqs, levels, noises, par.scalar<int>("Generate_One_Ref_Per_Noise_Level", 1) == 1,
+                                       par.scalar<double>("SPLRef", 2e-5), {}, {}, {});
+      for (size_t g = 0; g < lay.prof.size(); ++g) {
+        const cm_jnd_profile &q = lay.prof[g];
+        std::printf("%zu %.17g %.17g %.17g %d %d %d %d\n", g, q.dA, q.Wn, q.frequency, (int)q.isReference,
+                    q.calculateIndexPosition, q.referenceIndex, lay.serial[g]);
+      }
+      return 0;
+    }
     if (argc != 2) {
-      std::fprintf(stderr, "usage: %s <file.par> | --parse <file.par>\n", argv[0]);
+      std::fprintf(stderr, "usage: %s <file.par> | --parse <file.par> | --profiles <file.par>\n", argv[0]);
       return 2;
     }
     ParFile par(argv[1]);

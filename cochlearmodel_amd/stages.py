@@ -208,7 +208,7 @@ def jnd_profiles(freqs, levels, noises=(1111.0,), spl_ref=2e-5, one_ref_per_nois
                 prof[g]["dA"] = pw
                 prof[g]["recipdA"] = 1.0 / pw if pw > 0 else 0.0
                 prof[g]["Wn"] = npow
-                prof[g]["frequency"] = fr
+                prof[g]["frequency"] = float(np.float32(fr))       # testedFrequencies is float
                 calc += 1
                 g += 1
             if not one_ref_per_noise and calculate_jnd:

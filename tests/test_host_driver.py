@@ -259,3 +259,26 @@ def test_complex_jnd_run_files(tmp_path):
     np.testing.assert_allclose(got_fin[:val.size], val.reshape(-1), atol=1e-3)
     np.testing.assert_array_equal(got_fin[val.size:2 * val.size], warn.reshape(-1).astype(np.float32))
     np.testing.assert_array_equal(got_fin[2 * val.size:], conv.reshape(-1).astype(np.float32))
+
+
+@pytest.mark.parametrize("one_ref", [1, 0])
+def test_jnd_profile_layout_matches_python_mirror(tmp_path, one_ref):
+    """The host's CParams::updateInputProfile restatement (cochlea_run
+    --profiles) against stages.jnd_profiles: dA, noise power, frequency,
+    reference flags and indexes, serial positions."""
+    from cochlearmodel_amd import stages
+    p = tmp_path / "p.par"
+    p.write_text(f"testedFrequencies = 1000.3 3000\ntestedPowerLevels = 20 40.7 60\ntestedNoises = 1111 40\n"
+                 f"Generate_One_Ref_Per_Noise_Level = {one_ref}\n")
+    r = subprocess.run([EXE, "--profiles", str(p)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    rows = [line.split() for line in r.stdout.splitlines()]
+    prof, refs, serial, calc, to_ref = stages.jnd_profiles([1000.3, 3000.0], [20.0, 40.7, 60.0], noises=(1111.0, 40.0),
+                                                           one_ref_per_noise=bool(one_ref))
+    assert len(rows) == len(prof)
+    for g, row in enumerate(rows):
+        assert int(row[0]) == g
+        assert float(row[1]) == prof[g]["dA"] and float(row[2]) == prof[g]["Wn"]
+        assert float(row[3]) == prof[g]["frequency"]
+        assert int(row[4]) == prof[g]["isReference"] and int(row[5]) == prof[g]["calculateIndexPosition"]
+        assert int(row[6]) == prof[g]["referenceIndex"] and int(row[7]) == serial[g]
