@@ -106,16 +106,29 @@ __device__ __forceinline__ float in_at(const BmLaunch &a, int64_t idx) {
 // J Jacobi sweeps of the kernel convention (include/cochlea.cuh:13962-13982):
 // p_i <- (Y_i - (l_i p_{i-1} + p_{i+1})) / m_i with the lane-boundary
 // neighbours exchanged by DPP wave shifts (zero past either end).
+__device__ __forceinline__ void jacobi_sweep(const float (&Y)[4], const float (&rsm)[4], float low23,
+                                             const float (&p)[4], float (&q)[4]) {
+  const float left = wave_shr1(p[3]);
+  const float right = wave_shl1(p[0]);
+  q[0] = (Y[0] - (left + p[1])) * rsm[0];
+  q[1] = (Y[1] - (p[0] + p[2])) * rsm[1];
+  q[2] = (Y[2] - (low23 * p[1] + p[3])) * rsm[2];
+  q[3] = (Y[3] - (low23 * p[2] + right)) * rsm[3];
+}
+// J is wave-uniform: the count goes to a scalar register, and sweeps are
+// paired so the two iterates alternate between register sets instead of
+// being copied back each sweep.
 __device__ __forceinline__ void jacobi_sweeps(const float (&Y)[4], const float (&rsm)[4], float low23, float (&p)[4],
                                               int J) {
-  for (int k = 0; k < J; ++k) {
-    const float left = wave_shr1(p[3]);
-    const float right = wave_shl1(p[0]);
-    const float q0 = (Y[0] - (left + p[1])) * rsm[0];
-    const float q1 = (Y[1] - (p[0] + p[2])) * rsm[1];
-    const float q2 = (Y[2] - (low23 * p[1] + p[3])) * rsm[2];
-    const float q3 = (Y[3] - (low23 * p[2] + right)) * rsm[3];
-    p[0] = q0; p[1] = q1; p[2] = q2; p[3] = q3;
+  J = __builtin_amdgcn_readfirstlane(J);
+  float q[4];
+  for (int k = 1; k < J; k += 2) {
+    jacobi_sweep(Y, rsm, low23, p, q);
+    jacobi_sweep(Y, rsm, low23, q, p);
+  }
+  if (J & 1) {
+    jacobi_sweep(Y, rsm, low23, p, q);
+    UNROLL for (int j = 0; j < 4; ++j) p[j] = q[j];
   }
 }
 
