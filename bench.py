@@ -67,6 +67,15 @@ MEASURED_HBM_BYTES = {
     "ihc_lambda": (1151030.6 * 2.0 + 4515840.0 + 2289617.2 * 2.0 + 6773760.0) * 1024,
 }
 PEAK_BF16_TFLOPS = 2500.0    # MI355X dense BF16 MFMA
+# VALU instructions per launch on the 256-tone batch (rocprofv3 --pmc SQ_INSTS_VALU;
+# profiles/pmc_valu_c2_reference.md, profiles/pmc_mix_c2_learned.md) and the issue
+# capacity they are measured against: a wave64 FP32 VALU instruction holds a SIMD
+# for 4 cycles, 1024 SIMDs at 2.4 GHz
+MEASURED_VALU_INSTS = {"reference_20_2": 1.181e11, "learned_tol": 2.856e11}
+VALU_ISSUE_PER_S = 1024 * 2.4e9 / 4
+# L1<-L2 reads of the learned kernel: TCP_TCC_READ_REQ_sum x 128 B (a 1 GiB copy
+# reads 8.389e6 requests); profiles/pmc_mix_c2_learned.md
+MEASURED_L2_READ_BYTES = {"learned_tol": 8.048e10 * 128}
 
 
 def load_warm_start_model():
@@ -460,8 +469,17 @@ def main():
                                 "traffic": MEASURED_HBM_BYTES["ihc_lambda"] if args.signals == 256 else None,
                                 "fp64_flop_per_launch": work.rows * SECTIONS * 2.0 * (76 + 88)},
     }
+    if args.signals == 256 and args.variant in MEASURED_VALU_INSTS:
+        out["roofline"]["valu_issue"] = {
+            "insts_per_launch": MEASURED_VALU_INSTS[args.variant],
+            "frac": MEASURED_VALU_INSTS[args.variant] / (bm_avg * 1e-3) / VALU_ISSUE_PER_S,
+            "capacity_per_s": VALU_ISSUE_PER_S}
     if args.variant == "learned_tol":
         out["roofline"]["kernel"] = "bm_ws_kernel"
+        if args.signals == 256:
+            out["roofline"]["l2_read"] = {"bytes_per_launch": MEASURED_L2_READ_BYTES["learned_tol"],
+                                          "achieved": MEASURED_L2_READ_BYTES["learned_tol"] / (bm_avg * 1e-3) / 1e12,
+                                          "unit": "TB/s"}
         mlp = steps * MLP_FLOP_PER_STEP / (bm_avg * 1e-3) / 1e12
         out["mlp_roofline"] = {"bound": "mfma", "engine": "BF16 MFMA", "achieved": mlp, "peak": PEAK_BF16_TFLOPS,
                                "unit": "TFLOP/s", "frac": mlp / PEAK_BF16_TFLOPS,
