@@ -52,11 +52,20 @@ def variant_scalars(name: str):
     if name == "reference_20_2":
         return sc
     sc.jacobi_mode, sc.tol_rel, sc.check_every, sc.max_sweeps = 1, TOL, 4, 1000
-    sc.warm_start = {"plain_tol": 0, "extrapolated_tol": 1, "learned_tol": 2}[name]
+    sc.warm_start = 2 if name.startswith("learned_tol") else {"plain_tol": 0, "extrapolated_tol": 1}[name]
     return sc
 
 
+def set_network_every(name: str):
+    """The learned variants consult the network on every step (learned_tol) or
+    on every k-th step (learned_tol_every<k>), extrapolating in between."""
+    from cochlearmodel_amd import _lib
+    k = int(name.split("every")[1]) if "every" in name else 1
+    _lib.check(_lib.lib().cm_ws_set_every(k), "cm_ws_set_every")
+
+
 VARIANTS = ("reference_20_2", "plain_tol", "extrapolated_tol", "learned_tol")
+EXTRA_VARIANTS = ("learned_tol_every2", "learned_tol_every4")   # reported in the variants table only
 MLP_FLOP_PER_STEP = 2.0 * (769 * 512 + 512 * 512 + 512 * 256)   # forward, one block-step
 # HBM bytes per launch on the 256-tone batch from rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE (KiB per dispatch), scaled by what the counters read for a 1 GiB
@@ -492,9 +501,11 @@ def main():
         print(f"# exact-solve reference on {args.accuracy_blocks} blocks done", file=sys.stderr, flush=True)
         nrm = max(float(np.linalg.norm(exact)), 1e-30)
         var = {}
-        for name in VARIANTS:
+        for name in VARIANTS + EXTRA_VARIANTS:
             vsc = variant_scalars(name)
+            set_network_every(name)
             ms = work.run_bm(vsc, True)
+            set_network_every("learned_tol")
             torch.cuda.synchronize()
             st, pa, sw = work.stats()
             v = work.d_speeds.cpu().numpy()[rows]

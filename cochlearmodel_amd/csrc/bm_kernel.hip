@@ -639,6 +639,8 @@ __global__ __launch_bounds__(64 * G) void bm_ws_kernel(BmLaunch a, WsArgs w) {
   const float inv_256 = 1.0f / 256.0f;
   for (int round = 0;; ++round) {
     const bool active = valid && s.in_s < it.mesi;
+    // the network runs on every net_every-th round of the group (group-uniform)
+    const bool net_round = w.use_net && (round % w.net_every) == 0;
     float rr0 = 0.f, rre = 0.f, rr_scale = 0.f;
     if (active) {
       float od[4], G4[4], Y[4];
@@ -648,6 +650,7 @@ __global__ __launch_bounds__(64 * G) void bm_ws_kernel(BmLaunch a, WsArgs w) {
       float r[4];
       resid_vec(Y, k.mf, k.low23, s.p, r);
       rr_scale = __builtin_sqrtf(rre * inv_256) + 1e-30f;
+      if (net_round || w.capture) {
       const float4 pk = pold[lane];   // plain guess (previous pressure)
       const float inv_d = 1.0f / rms256(s.nd[0], s.nd[1], s.nd[2], s.nd[3]);
       const float inv_v = 1.0f / rms256(s.nv[0], s.nv[1], s.nv[2], s.nv[3]);
@@ -681,6 +684,7 @@ __global__ __launch_bounds__(64 * G) void bm_ws_kernel(BmLaunch a, WsArgs w) {
           rec[771] = 0.f;
         }
       }
+      }
     }
     // a fast wave may write the next round's flag before a slow one has read
     // this round's, so the flags alternate between two slots
@@ -689,7 +693,7 @@ __global__ __launch_bounds__(64 * G) void bm_ws_kernel(BmLaunch a, WsArgs w) {
     int any = 0;
     for (int g = 0; g < G; ++g) any |= active_lds[round & 1][g];
     if (!any) break;
-    if (w.use_net) {
+    if (net_round) {
       // X -> H2 -> H1 -> output tile in H2's space (H2 is dead once layer 2 ends)
       typedef const CM_LDS uint16_t *lds_in;
       typedef CM_LDS void *lds_out;
@@ -714,7 +718,7 @@ __global__ __launch_bounds__(64 * G) void bm_ws_kernel(BmLaunch a, WsArgs w) {
       pass_begin(s, true);
       const float sf = pass_rhs(k, c, s, true, lane, od, G4, Y);
       const float yy = tree_sum(Y[0] * Y[0], Y[1] * Y[1], Y[2] * Y[2], Y[3] * Y[3]);
-      if (w.use_net) {
+      if (net_round) {
         // learned guess: p_lin + rms(r_lin) c_out S(y), S = [1,2,1]/4 (edges replicated)
         const float4 y4 = *reinterpret_cast<const float4 *>(reinterpret_cast<const float *>(H2s) + wave * WS_OS +
                                                              4 * lane);
@@ -804,7 +808,7 @@ extern "C" int cm_launch_bm(const cm::BmLaunch *a, int waves_per_block, int grid
 // Lockstep groups of G blocks: the smallest G in {8, 10, 12} whose groups fit
 // one per CU, else 12.  Returns G (> 0) or a negative error.
 extern "C" int cm_launch_bm_ws(const cm::BmLaunch *a, const cm::WsArgs *w, int n_cus, hipStream_t stream) {
-  if (a->sc.jacobi_mode != 1) return -1;
+  if (a->sc.jacobi_mode != 1 || w->net_every < 1) return -1;
   const int n = a->n_items;
   int G = 12;
   if ((n + 7) / 8 <= n_cus) G = 8;

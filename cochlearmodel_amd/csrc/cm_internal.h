@@ -46,6 +46,7 @@ struct WsArgs {
   float *records;                    // capture: [items * per_item][CM_WS_RECORD] floats
   int32_t stride;                    // capture every stride-th committed step
   int32_t per_item;                  // capture slots per item
+  int32_t net_every;                 // evaluate the network on every net_every-th round (>= 1)
 };
 
 // record layout (floats): features [0, 769), rms(r_lin) at 769, committed step

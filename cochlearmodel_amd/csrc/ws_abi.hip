@@ -22,6 +22,7 @@ struct WsHolder {
   float *records = nullptr;   // capture target (caller-owned device memory)
   int64_t n_records = 0;
   int stride = 0, per_item = 0;
+  int net_every = 1;          // network on every net_every-th lockstep round
   double *ainv = nullptr;     // 256 x 256 inverse of the kernel-convention matrix (targets)
 };
 WsHolder g_ws;
@@ -103,6 +104,7 @@ int ws_args(const cm_bm_scalars &sc, int n_items, WsArgs *out) {
     out->c_out = g_ws.c_out;
     out->use_net = 1;
   }
+  out->net_every = g_ws.net_every;
   if (g_ws.records) {
     if ((int64_t)n_items * g_ws.per_item > g_ws.n_records) {
       set_error("capture buffer holds %lld records, %d items x %d slots need more", (long long)g_ws.n_records,
@@ -178,6 +180,18 @@ int cm_ws_load_file(const char *path) {
     p += n[i];
   }
   return cm_ws_load(parts[0], parts[1], parts[2], parts[3], parts[4], parts[5], *p);
+}
+
+// Evaluate the warm-start network on every k-th lockstep round only; the other
+// rounds take the better of the previous and the extrapolated pressure, as
+// warm_start = 1 does.  k = 1 (default) uses the network on every step.
+int cm_ws_set_every(int k) {
+  if (k < 1 || k > 1000000) {
+    set_error("cm_ws_set_every: k must be >= 1");
+    return -3;
+  }
+  g_ws.net_every = k;
+  return 0;
 }
 
 int cm_ws_set_capture(float *d_records, int64_t n_records, int stride, int per_item) {

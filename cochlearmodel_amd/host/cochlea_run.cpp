@@ -44,8 +44,9 @@
 // Keys of this library's solver options (no reference counterpart):
 // Jacobi_Tolerance (> 0: Jacobi to that relative residual instead of the fixed
 // sweeps), Jacobi_Check_Every (4), Jacobi_Max_Sweeps (1000), Warm_Start (0
-// previous pressure, 1 extrapolated, 2 learned) and Warm_Start_Model (default
-// data/warmstart_mlp.bin beside this executable).
+// previous pressure, 1 extrapolated, 2 learned), Warm_Start_Model (default
+// data/warmstart_mlp.bin beside this executable) and Warm_Start_Every (the
+// network on every k-th step, default 1).
 #include <algorithm>
 #include <cctype>
 #include <chrono>
@@ -331,8 +332,10 @@ int run(const ParFile &par, const std::string &exe_dir) {
   const int DF = par.scalar<int>("Decouple_Filter", 0);
   const float tau = par.scalar<float>("Jacobi_Tolerance", 0.f);
   const int warm = par.scalar<int>("Warm_Start", 0);
-  if (warm == 2)
+  if (warm == 2) {
     ok(cm_ws_load_file(par.str("Warm_Start_Model", exe_dir + "/data/warmstart_mlp.bin").c_str()), "cm_ws_load_file");
+    ok(cm_ws_set_every(par.scalar<int>("Warm_Start_Every", 1)), "cm_ws_set_every");
+  }
   ok(cm_set_bm_solver(tau > 0.f ? 1 : 0, tau > 0.f ? tau : 1e-4f, par.scalar<int>("Jacobi_Check_Every", 4),
                       par.scalar<int>("Jacobi_Max_Sweeps", 1000), warm),
      "cm_set_bm_solver");

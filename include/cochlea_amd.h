@@ -307,6 +307,10 @@ int cm_ws_load(const float *W1, const float *b1, const float *W2, const float *b
  * rms(r_lin) (769), the step index (770), Y (256 at 772), p_lin (256 at 1028)
  * and, with warm_start 2, the network output (256 at 1288); 1544 floats per
  * record.  d_records = NULL switches capture off. */
+/* run the warm-start network on every k-th lockstep round only (k >= 1; 1 = every
+ * step); other rounds take the better of the previous and extrapolated pressure
+ * (no reference counterpart) */
+int cm_ws_set_every(int k);
 int cm_ws_set_capture(float *d_records, int64_t n_records, int stride, int per_item);
 /* cm_ws_load from a file of float32: W1, b1, W2, b2, W3, b3 (as above) and c_out */
 int cm_ws_load_file(const char *path);

@@ -158,11 +158,21 @@ def test_trained_network_gains_over_plain_and_extrapolated(oracle):
     warmstart.upload(params, c_out)
     _say("case: trained network, 8 held-out tones x 100 ms")
     tb = _batch(n=8, dur=0.1, seed=warmstart.EVAL_SEED + 1000)
+    from cochlearmodel_amd import _lib
     res = {}
-    for name, ws in (("plain", 0), ("extrapolated", 1), ("learned", 2)):
+    for name, ws, every in (("plain", 0, 1), ("extrapolated", 1, 1), ("learned", 2, 1), ("learned_every2", 2, 2),
+                            ("learned_again", 2, 1)):
+        _lib.check(_lib.lib().cm_ws_set_every(every))
         v, st = _run(tb, warmstart.tol_scalars(1e-6, ws))
         res[name] = (v, st[:, 2].sum() / st[:, 0].sum(), st[:, 0].sum())
         _say(f"{name}: {res[name][1]:.2f} sweeps/step")
+    _lib.check(_lib.lib().cm_ws_set_every(1))
+    # the network on every other step lands between the two; resetting restores
+    # the every-step results exactly
+    assert res["learned"][1] < res["learned_every2"][1] < res["extrapolated"][1]
+    np.testing.assert_array_equal(res["learned_again"][0], res["learned"][0])
+    with pytest.raises(_lib.CochleaError):
+        _lib.check(_lib.lib().cm_ws_set_every(0))
     sel = tb.sample_blocks(8)
     it = np.zeros(len(sel), oracle.ITEM_DTYPE)
     for f in ("start", "mosi", "mesi", "write_start", "tol", "m1tol", "set"):
@@ -177,4 +187,5 @@ def test_trained_network_gains_over_plain_and_extrapolated(oracle):
     _say(f"rel L2 vs exact solve: {err}")
     assert res["learned"][1] < res["extrapolated"][1] < res["plain"][1]
     assert err["learned"] <= max(1.5 * err["plain"], 1e-3)
+    assert err["learned_every2"] <= max(1.5 * err["plain"], 1e-3)
     assert abs(int(res["learned"][2]) - int(res["plain"][2])) <= 0.01 * res["plain"][2]
