@@ -20,8 +20,9 @@
 // emitted rows of a first chunk are [overlap, totalTimeNodes) (the transient
 // overlap is discarded, src/solver.cpp:704-727).
 //
-// Supported keys: Fs, Input_Signal, Input_File (raw float64, CBin), Sin_Frequency
-// and Sin_dB (generated tone when no input is given), Duration, Time_Blocks,
+// Supported keys: Fs, Sim_Type (0 VOICE, 1 sine, 3/4 JND runs below),
+// Input_Signal, Input_File (raw float64, CBin), Sin_Freq and Sin_dB (the tone
+// of Sim_Type 1), Duration, Time_Blocks,
 // Time_Block_Length, Overlap_Time, OHC_Vector, IHC_Vector, Allowed_Outputs,
 // Discard_BM_Velocity_Output, Discard_Lambdas_Output, Output_Result,
 // JACOBBY_Loops_Fast, JACOBBY_Loops_Slow, Cuda_Outern_Loops, MAX_M1_SP_ERROR,
@@ -282,15 +283,16 @@ int run(const ParFile &par, const std::string &exe_dir) {
     signal = par.vec<double>("Input_Signal");
   } else if (par.has("Input_File") && !par.str("Input_File").empty()) {
     signal = read_float64_file(par.str("Input_File"));
-  } else if (par.has("Sin_Frequency") && par.has("Sin_dB")) {
-    const double f = par.scalar<double>("Sin_Frequency", 1000.0), db = par.scalar<double>("Sin_dB", 60.0);
-    const double amp = 10.0 * 20e-6 * std::pow(10.0, db / 20.0);   // CONV_dB_TO_AMP, include/const.h:53
-    const int n = (int)std::lround(par.scalar<double>("Duration", 0.1) * Fs);
-    signal.resize(n);
-    for (int i = 0; i < n; ++i) signal[i] = amp * std::sin(2.0 * M_PI * f * i / Fs);
+  } else if (par.scalar<int>("Sim_Type", 0) == 1 || (par.has("Sin_Freq") && par.has("Sin_dB"))) {
+    // SIM_TYPE_SIN: CState::gen_input_data (src/state.cpp:411-435) from time 0,
+    // amplitude CONV_dB_TO_AMP(Sin_dB) in float (include/const.h:53), the
+    // sample time accumulated in steps of 1/Fs; generated below for the
+    // whole buffer
   } else {
-    throw std::runtime_error("no input: set Input_Signal, Input_File or Sin_Frequency/Sin_dB");
+    throw std::runtime_error("no input: set Input_Signal, Input_File, or Sin_Freq/Sin_dB (Sim_Type 1)");
   }
+  const bool sine = run_stage == 0 && signal.empty() &&
+                    (par.scalar<int>("Sim_Type", 0) == 1 || (par.has("Sin_Freq") && par.has("Sin_dB")));
   int TB = par.scalar<int>("Time_Blocks", 0);
   if (TB <= 0) {
     const double duration = par.scalar<double>("Duration", (double)signal.size() / Fs);
@@ -324,6 +326,16 @@ int run(const ParFile &par, const std::string &exe_dir) {
      "BMOHCKernel_Init");
 
   std::vector<float> input(bufP1, 0.f);
+  if (sine) {
+    const float freq = par.scalar<float>("Sin_Freq", 1000.f), db = par.scalar<float>("Sin_dB", 60.f);
+    const float amp = 10.0f * 20.0e-6f * std::pow(10.0f, db / 20.0f);
+    const double Ts = 1.0 / (double)Fs;
+    double t = 0.0;
+    for (int i = 0; i < bufP1; ++i) {
+      input[i] = (float)((double)amp * std::sin(2.0 * M_PI * (double)freq * t));
+      t += Ts;
+    }
+  }
   for (int i = 0; i < bufP1 && i < (int)signal.size(); ++i) input[i] = (float)signal[i];
   cm_bm_scalars sc;
   ok(cm_default_bm_scalars(Fs, &sc), "cm_default_bm_scalars");

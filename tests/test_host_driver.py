@@ -282,3 +282,33 @@ def test_jnd_profile_layout_matches_python_mirror(tmp_path, one_ref):
         assert float(row[3]) == prof[g]["frequency"]
         assert int(row[4]) == prof[g]["isReference"] and int(row[5]) == prof[g]["calculateIndexPosition"]
         assert int(row[6]) == prof[g]["referenceIndex"] and int(row[7]) == serial[g]
+
+
+@pytest.mark.gpu
+def test_sine_run(tmp_path, oracle):
+    """Sim_Type 1 (SIM_TYPE_SIN): the tone of CState::gen_input_data (time
+    accumulated in steps of 1/Fs, amplitude CONV_dB_TO_AMP in float) through
+    the BM stage, bit for bit against the oracle on the same input."""
+    import math
+    from cochlearmodel_amd import solver
+    lay = solver.voice_layout(FS, 5, 0.02, 0.005)
+    TB, ov, tot, bufP1 = lay.time_blocks, lay.overlap_nodes, lay.total_nodes, lay.bufP1
+    print("case: cochlea_run Sim_Type 1, 1 kHz at 60 dB, 5 time blocks", flush=True)
+    out = tmp_path / "sin.bin"
+    (tmp_path / "sin.par").write_text(f"Sim_Type = 1\nSin_Freq = 1000\nSin_dB = 60\nFs = 44100\nTime_Blocks = 5\n"
+                                      f"Allowed_Outputs = 1\nOutput_Result = {out}\n")
+    r = subprocess.run([EXE, str(tmp_path / "sin.par")], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    amp = float(np.float32(10.0) * np.float32(20e-6) * np.power(np.float32(10.0), np.float32(60.0) / np.float32(20.0)))
+    x = np.zeros(bufP1, np.float32)
+    t, Ts = 0.0, 1.0 / FS
+    for i in range(bufP1):
+        x[i] = np.float32(amp * math.sin(2.0 * math.pi * 1000.0 * t))
+        t += Ts
+    st, mo, me, ws = oracle.block_decomposition(TB, bufP1, ov, 0)
+    o = oracle.bm_run(oracle.default_scalars(FS), oracle.section_params(), np.full(256, 0.5, np.float32),
+                      oracle.make_items(st, mo, me, ws, 1e-10, 1e-15), x, bufP1, "f32", threads=8)
+    got = np.fromfile(out, np.float32)
+    n = tot - ov
+    v = got[n + TB:n + TB + n * 256]                # after failed nodes and blocks
+    np.testing.assert_array_equal(v, o["speeds"][ov:tot].ravel())
