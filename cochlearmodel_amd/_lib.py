@@ -52,6 +52,12 @@ _SIGS = {
     "cm_last_error": (C.c_char_p, []),
     "cm_set_stream": (C.c_int, [vp]),
     "cm_device_synchronize": (C.c_int, []),
+    "viewGPUStatus": (C.c_int, [C.c_int, C.c_char_p]),
+    "cudaEventsCreate": (C.c_int, [vp, vp, C.c_int]),
+    "cudaEventsStartTimer": (C.c_int, [vp, vp, C.c_int]),
+    "cudaEventsStopTimer": (C.c_int, [vp, vp, C.c_int]),
+    "cudaEventsStopQueryTimer": (f32, [vp, vp, C.c_int, C.c_char_p]),
+    "cudaEventsDestroy": (C.c_int, [vp, vp]),
     "cm_last_kernel_ms": (C.c_float, []),
     "cm_set_bm_launch": (C.c_int, [C.c_int, C.c_int]),
     "cm_set_bm_order": (C.c_int, [vp, C.c_int]),

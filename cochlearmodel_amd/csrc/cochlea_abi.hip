@@ -229,6 +229,53 @@ int cm_set_stream(void *s) {
 
 int cm_device_synchronize(void) { return check(hipDeviceSynchronize(), "hipDeviceSynchronize"); }
 
+// Diagnostics of src/cochlea.cu:708-746 with plain-C handles: the events are
+// hipEvent_t values passed as void*, the prefix a C string.
+int viewGPUStatus(int flags, const char *prefix) {
+  if (!(flags & 16)) return 0;
+  size_t free_b = 0, total_b = 0;
+  if (check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo")) return -1;
+  std::printf("%s : GPU Memory, Free(%d MB) / Total(%d MB)\n", prefix ? prefix : "", (int)(free_b >> 20),
+              (int)(total_b >> 20));
+  return 0;
+}
+int cudaEventsCreate(void **start, void **stop, int condition) {
+  if (!condition) return 0;
+  if (!start || !stop) {
+    set_error("cudaEventsCreate: null handle");
+    return -3;
+  }
+  hipEvent_t a, b;
+  if (check(hipEventCreate(&a), "hipEventCreate") || check(hipEventCreate(&b), "hipEventCreate")) return -1;
+  *start = a;
+  *stop = b;
+  return 0;
+}
+int cudaEventsStartTimer(void *start, void *stop, int condition) {
+  (void)stop;
+  return condition ? check(hipEventRecord((hipEvent_t)start, stream()), "hipEventRecord") : 0;
+}
+int cudaEventsStopTimer(void *start, void *stop, int condition) {
+  (void)start;
+  return condition ? check(hipEventRecord((hipEvent_t)stop, stream()), "hipEventRecord") : 0;
+}
+// returns the elapsed milliseconds (and prints them as the reference does), or
+// -1 on an error
+float cudaEventsStopQueryTimer(void *start, void *stop, int condition, const char *prefix) {
+  if (!condition) return 0.0f;
+  float ms = 0.0f;
+  if (cudaEventsStopTimer(start, stop, condition) || check(hipEventSynchronize((hipEvent_t)stop), "hipEventSynchronize") ||
+      check(hipEventElapsedTime(&ms, (hipEvent_t)start, (hipEvent_t)stop), "hipEventElapsedTime"))
+    return -1.0f;
+  std::printf("%s : %.2f (msec) \n", prefix ? prefix : "", ms);
+  return ms;
+}
+int cudaEventsDestroy(void *start, void *stop) {
+  if (start) (void)hipEventDestroy((hipEvent_t)start);
+  if (stop) (void)hipEventDestroy((hipEvent_t)stop);
+  return 0;
+}
+
 float cm_last_kernel_ms(void) { return g_bm.last_kernel_ms; }
 
 // Host-side longest-first order of a block table (stable for equal lengths).

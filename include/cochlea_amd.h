@@ -47,6 +47,15 @@ typedef struct {
 const char *cm_last_error(void);
 int cm_set_stream(void *hip_stream);
 int cm_device_synchronize(void);
+/* replace viewGPUStatus and the cudaEvents* timers (include/cochlea_common.h:96-99,
+ * 289; src/cochlea.cu:708-746): events are hipEvent_t handles passed as void*,
+ * the prefix a C string; cudaEventsDestroy releases a pair */
+int viewGPUStatus(int flags, const char *prefix);
+int cudaEventsCreate(void **start, void **stop, int condition);
+int cudaEventsStartTimer(void *start, void *stop, int condition);
+int cudaEventsStopTimer(void *start, void *stop, int condition);
+float cudaEventsStopQueryTimer(void *start, void *stop, int condition, const char *prefix);
+int cudaEventsDestroy(void *start, void *stop);
 /* Duration of the last timed BM launch (Show_Run_Time & 1, or timed=1). */
 float cm_last_kernel_ms(void);
 /* BM launch shape: waves (time blocks) per workgroup, persistent queue. */

@@ -294,3 +294,24 @@ def test_lambda_fix_up_kernel():
     s = sp[:, None, None]
     np.testing.assert_array_equal(d_lam.cpu().numpy(), np.maximum(np.float32(0), lam - s))
     np.testing.assert_array_equal(d_buf.cpu().numpy(), np.maximum(s, lam))
+
+
+def test_diagnostic_entry_points():
+    """viewGPUStatus and the cudaEvents* timers with plain-C handles."""
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import ctypes as C
+    from cochlearmodel_amd import _lib
+    L = _lib.lib()
+    _say("case: diagnostics (memory status, event timer)")
+    assert L.viewGPUStatus(16, b"diagnostics test") == 0
+    a, b = C.c_void_p(), C.c_void_p()
+    _lib.check(L.cudaEventsCreate(C.byref(a), C.byref(b), 1))
+    _lib.check(L.cudaEventsStartTimer(a, b, 1))
+    x = torch.ones(1 << 20, device="cuda")
+    x.mul_(2.0)
+    ms = L.cudaEventsStopQueryTimer(a, b, 1, b"timer")
+    assert ms >= 0.0
+    _lib.check(L.cudaEventsDestroy(a, b))
+    assert L.cudaEventsStopQueryTimer(None, None, 0, b"off") == 0.0
