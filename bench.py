@@ -291,13 +291,16 @@ def c4_main(args):
     sc = solver.default_scalars(FS)
     print(f"# JND sweep ready: {sw.n_prof} intervals, {sw.TB} time blocks, {sw.rows} rows", file=sys.stderr,
           flush=True)
+    use_graph = not args.no_graph
+    if use_graph:
+        sw.build_graph()          # IHC/lambda + JND reductions captured as one HIP graph
     for _ in range(args.warmup):
-        sw.step(sc)
+        sw.step(sc, use_graph=use_graph)
     torch.cuda.synchronize()
     times = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        times.append(sw.step(sc))
+        times.append(sw.step(sc, use_graph=use_graph))
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     th0 = time.perf_counter()
@@ -306,6 +309,11 @@ def c4_main(args):
     steps, passes, sweeps = sw.stats()
     ms_per_step = 1e3 * wall / args.steps
     avg = {k: float(np.mean([t[k] for t in times])) for k in times[0]}
+    if use_graph:
+        # one eager step (untimed for the value) gives the per-stage split
+        eager = sw.step(sc)
+        avg["ihc_ms"], avg["jnd_ms"] = eager["ihc_ms"], eager["jnd_ms"]
+        sw.release_graph()
     flops = flops_per_launch(steps, passes, sweeps)
     bm_tf = flops / (avg["bm_ms"] * 1e-3) / 1e12
     lam_bytes = sw.rows * SECTIONS * 16.0
@@ -318,7 +326,8 @@ def c4_main(args):
         "config": {"workload": "JND sweep: 16 tones log-spaced 250-8000 Hz x 10 levels 10-100 dB SPL x 5 noise "
                                "conditions (none; seeded white noise at 20/40/60/80 dB SPL), 300 ms intervals at "
                                "44.1 kHz, one reference interval per noise condition, Decouple_Filter=15, "
-                               "reference kernel-17 method; input synthesis + BM + IHC/lambda + JND per step",
+                               "reference kernel-17 method; input synthesis + BM + IHC/lambda + JND per step "
+                               "(IHC/lambda + JND replayed as one captured HIP graph unless --no-graph)",
                    "intervals": sw.n_prof, "time_blocks": sw.TB, "rows": sw.rows},
         "stage_ms": avg, "host_final_and_complex_ms": host_ms, "committed_steps": steps,
         "jacobi_iters_per_step": sweeps / steps, "profiles_per_s": sw.n_prof / (ms_per_step * 1e-3),
@@ -400,6 +409,7 @@ def main():
                     help="c2: 256 uniform tones (headline); c3: 1024 uneven signals; c4: JND sweep")
     ap.add_argument("--c3-signals", type=int, default=1024)
     ap.add_argument("--c4-intervals", type=int, default=8, help="calculated intervals checked on the CPU")
+    ap.add_argument("--no-graph", action="store_true", help="c4: launch the post-BM stages eagerly")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (calibrates HBM byte counters)")
     args = ap.parse_args()

@@ -117,6 +117,11 @@ _SIGS = {
                                vp, vp, vp]),
     "cm_remez_lowpass": (C.c_int, [C.c_int, f64, f64, f64, f64, C.c_int, vp, vp, vp]),
     "cm_design_ac_filter": (C.c_int, [f64, C.c_char_p, vp, vp, C.c_int, vp, vp]),
+    "cm_post_bm_graph_create": (C.c_int, [vp, C.c_int64, vp, C.c_int, vp, C.c_int, vp, C.c_int, vp, vp, vp, f32,
+                                          f32, f32, f32, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, vp,
+                                          C.c_int, vp, vp, f64, f64, C.c_int, C.c_int, vp, vp, vp, vp]),
+    "cm_post_bm_graph_launch": (C.c_int, [vp]),
+    "cm_post_bm_graph_destroy": (C.c_int, [vp]),
     "cm_jnd_failed_signals": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp, C.c_int, vp, C.c_int, vp]),
     "InputProfilesArrayInitialization": (C.c_int, [C.c_int] * 5),
     "InputProfilesArrayTermination": (C.c_int, []),

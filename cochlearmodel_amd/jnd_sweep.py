@@ -112,6 +112,31 @@ class JndSweep:
                                lib.ptr(self.d_nIHC), self.W3, FS, DELTA_ALPHA_TIME_FACTOR * FS, 1, 1,
                                lib.ptr(self.d_mean_rate), lib.ptr(self.d_F_RA), lib.ptr(self.d_FAI)), "cm_jnd_run")
 
+    # -- the post-BM stages as one HIP graph --------------------------------
+    def build_graph(self):
+        """Capture IHC/lambda + JND reductions once (cm_post_bm_graph_create)."""
+        L, lib = self.L, _lib
+        ivs = np.ascontiguousarray(self.fir_ivs, _lib.IHC_INTERVAL_DTYPE)
+        h = C.c_void_p()
+        lib.check(L.cm_post_bm_graph_create(
+            lib.ptr(self.d_speeds), self.rows, lib.ptr(ivs), len(ivs), lib.ptr(self.ihc.b), self.ihc.n_ac,
+            lib.ptr(self.ihc.dc), self.ihc.n_dc, lib.ptr(self.ihc.damage), lib.ptr(self.ihc.A), self.ihc.spont,
+            stages.LAMBDA_SAT, stages.ETA_AC, stages.ETA_DC, 1.0, lib.ptr(self.d_scratch), lib.ptr(self.d_lambda),
+            self.n_prof, self.interval, self.head, self.length, lib.ptr(self.d_dA), lib.ptr(self.d_ref_index),
+            lib.ptr(self.d_out_index), self.n_prof, lib.ptr(self.d_nIHC), self.W3, FS, DELTA_ALPHA_TIME_FACTOR * FS,
+            1, 1, lib.ptr(self.d_mean_rate), lib.ptr(self.d_F_RA), lib.ptr(self.d_FAI), C.byref(h)),
+            "cm_post_bm_graph_create")
+        self.graph = h
+
+    def run_graph(self):
+        self.L.cm_set_stream(self.stream)
+        _lib.check(self.L.cm_post_bm_graph_launch(self.graph), "cm_post_bm_graph_launch")
+
+    def release_graph(self):
+        if getattr(self, "graph", None) is not None:
+            self.L.cm_post_bm_graph_destroy(self.graph)
+            self.graph = None
+
     def finish(self):
         """Final dB and failed-signal counts per calculated interval (host, as
         calcJNDFinal) and the complex-JND aggregation (rate JND,
@@ -126,8 +151,10 @@ class JndSweep:
                                    self.eps, failed)
         return {"ai_db": ai[:n_calc], "rate_db": rate[:n_calc], "failed_signals": failed, "complex": final}
 
-    def step(self, sc, timed_bm: bool = True):
-        """Input synthesis + BM + IHC/lambda + JND reductions; per-stage ms."""
+    def step(self, sc, timed_bm: bool = True, use_graph: bool = False):
+        """Input synthesis + BM + IHC/lambda + JND reductions; per-stage ms.
+        With use_graph the post-BM stages are one replay of the captured graph
+        (build_graph first)."""
         torch = self.torch
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
         t = {}
@@ -136,15 +163,22 @@ class JndSweep:
         ev[1].record()
         ms = self.run_bm(sc, timed_bm)
         ev[2].record()
-        self.run_lambda()
-        ev[3].record()
-        self.run_jnd()
+        if use_graph:
+            self.run_graph()
+            ev[3].record()
+        else:
+            self.run_lambda()
+            ev[3].record()
+            self.run_jnd()
         ev[4].record()
         ev[4].synchronize()
         t["input_ms"] = ev[0].elapsed_time(ev[1])
         t["bm_ms"] = ms if timed_bm else ev[1].elapsed_time(ev[2])
-        t["ihc_ms"] = ev[2].elapsed_time(ev[3])
-        t["jnd_ms"] = ev[3].elapsed_time(ev[4])
+        if use_graph:
+            t["post_bm_graph_ms"] = ev[2].elapsed_time(ev[4])
+        else:
+            t["ihc_ms"] = ev[2].elapsed_time(ev[3])
+            t["jnd_ms"] = ev[3].elapsed_time(ev[4])
         return t
 
     def stats(self):

@@ -216,6 +216,22 @@ int cm_jnd_final(const float *F_RA, const float *FisherAISum, int JNDIntervalsFu
 int cm_jnd_failed_signals(const int32_t *failed_blocks, int n_blocks, int JNDIntervals, int handled, const int *refs,
                           int nrefs, const int *serial, int decouple_filter, int32_t *failed_signals);
 
+/* Post-BM stages as one HIP graph (no reference counterpart): IHC/lambda with a
+ * FIR AC filter (arguments as cm_ihc_lambda_run) and the JND reductions
+ * (arguments as cm_jnd_run) captured once; replays read the BM velocity at
+ * d_bm and rewrite lambda, mean rate and the Fisher sums.  Every device pointer
+ * must stay valid while the graph exists. */
+int cm_post_bm_graph_create(const float *d_bm, int64_t rows, const cm_ihc_interval *h_ivs, int n_ivs,
+                            const double *d_ac_b, int n_ac, const double *d_dc, int n_dc, const double *d_damage,
+                            const float *d_A, const float *spont3, float lambda_sat, float eta_ac, float eta_dc,
+                            float scale_bm, float *d_scratch, float *d_lambda, int n_intervals, int interval_nodes,
+                            int head_nodes, int length_nodes, const float *d_dA, const int32_t *d_ref_index,
+                            const int32_t *d_out_index, int out_stride, const double *d_nIHC, const float *W3,
+                            double Fs, double delta_alpha_factor, int calc_ai, int calc_rms, float *d_mean_rate,
+                            float *d_F_RA, float *d_FisherAISum, void **handle);
+int cm_post_bm_graph_launch(void *handle);
+int cm_post_bm_graph_destroy(void *handle);
+
 /* Complex-JND aggregation (Sim_Type 4), ComplexJNDProfile::calculateMinValue /
  * calculateGradientMinMaxValue (src/ComplexJNDProfile.cpp:19-95) as driven by
  * AudioGramCreator::calcComplexJND (src/AudioGramCreator.cpp:548-571).

@@ -99,3 +99,35 @@ def test_bench_interval_oracle_matches_whole_sweep(oracle, small_sweep):
         np.testing.assert_array_equal(speeds, o["speeds"][rs], err_msg=f"interval {j}")
         np.testing.assert_array_equal(lam, lo[:, rs], err_msg=f"lambda of interval {j}")
         assert steps > 0
+
+
+def test_post_bm_graph_replays_match_eager(small_sweep):
+    """The IHC/lambda + JND stages captured as one HIP graph: three replays,
+    with eager work (fills of the outputs, a BM rerun) between them, each give
+    the eager stages' lambda, mean rate and Fisher sums bit for bit."""
+    torch = _cuda()
+    from cochlearmodel_amd import solver
+    sw = small_sweep
+    sw.run_lambda()
+    sw.run_jnd()
+    torch.cuda.synchronize()
+    ref = [x.cpu().numpy().copy() for x in (sw.d_lambda, sw.d_mean_rate, sw.d_F_RA, sw.d_FAI)]
+    sw.build_graph()
+    try:
+        for rep in range(3):
+            _say(f"graph replay {rep}")
+            for x in (sw.d_lambda, sw.d_mean_rate, sw.d_F_RA, sw.d_FAI):
+                x.zero_()
+            if rep == 1:
+                sw.run_bm(solver.default_scalars(FS), timed=False)
+            torch.cuda.synchronize()
+            sw.run_graph()
+            torch.cuda.synchronize()
+            got = [x.cpu().numpy() for x in (sw.d_lambda, sw.d_mean_rate, sw.d_F_RA, sw.d_FAI)]
+            for g, r in zip(got, ref):
+                np.testing.assert_array_equal(g, r)
+        t = sw.step(solver.default_scalars(FS), timed_bm=False, use_graph=True)
+        assert t["post_bm_graph_ms"] > 0
+        np.testing.assert_array_equal(sw.d_F_RA.cpu().numpy(), ref[2])
+    finally:
+        sw.release_graph()
