@@ -113,15 +113,33 @@ def block_rows(work, sel):
 
 def exact_reference(work, n_blocks, threads):
     """O2 (exact per-pass pressure solve, FP32 state) on sampled blocks: the
-    accuracy yardstick every variant is scored against."""
+    accuracy yardstick every variant is scored against; also the FP64 rebuild
+    of the reference method on the same blocks."""
     from oracle import oracle as O
     sel = sample_blocks(work, n_blocks)
     sc = O.default_scalars(FS)
     sc.jacobi_mode = 2
-    o = O.bm_run(sc, O.section_params(), np.full(SECTIONS, 0.5, np.float32), oracle_items(work, sel),
-                 work.d_input.cpu().numpy(), work.bufP1, "f32", threads=threads)
+    inp = work.d_input.cpu().numpy()
+    o = O.bm_run(sc, O.section_params(), np.full(SECTIONS, 0.5, np.float32), oracle_items(work, sel), inp, work.bufP1,
+                 "f32", threads=threads)
+    o64 = O.bm_run(O.default_scalars(FS), O.section_params(), np.full(SECTIONS, 0.5, np.float32),
+                   oracle_items(work, sel), inp, work.bufP1, "f64", threads=threads)
     rows = block_rows(work, sel)
-    return rows, o["speeds"][rows]
+    return rows, o["speeds"][rows], o64["speeds"][rows]
+
+
+def host_cpu():
+    """Model name and core count of the host the CPU baseline ran on."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count()}
 
 
 def cpu_baseline(work, n_blocks: int, threads: int, sc_gpu):
@@ -243,7 +261,8 @@ def c3_main(args):
                                "sample": f"{len(sel)} of {ub.TB} time blocks (every {max(1, ub.TB // args.cpu_blocks)}"
                                          f"th), {st} committed steps, {dt:.2f} s wall, FP32 oracle, OpenMP over "
                                          "blocks",
-                               "parity_rel_l2": float(np.linalg.norm(g - ref) / max(np.linalg.norm(ref), 1e-30))}
+                               "parity_rel_l2": float(np.linalg.norm(g - ref) / max(np.linalg.norm(ref), 1e-30)),
+                               **host_cpu()}
     print(json.dumps(out))
 
 
@@ -389,7 +408,7 @@ def c4_main(args):
                                          f"{dt:.2f} s wall, FP32 oracle, OpenMP over blocks and sections",
                                "parity_rel_l2_bm": (dv / max(nv, 1e-300)) ** 0.5,
                                "parity_rel_l2_lambda": (dl / max(nl, 1e-300)) ** 0.5,
-                               "parity_max_jnd_db": dmax}
+                               "parity_max_jnd_db": dmax, **host_cpu()}
     print(json.dumps(out))
 
 
@@ -507,7 +526,8 @@ def main():
     threads = min(args.cpu_threads, os.cpu_count() or 1)
     if not args.no_variants:
         # every solver variant on the same resident input: BM kernel time, work and accuracy
-        rows, exact = exact_reference(work, args.accuracy_blocks, threads)
+        rows, exact, fp64 = exact_reference(work, args.accuracy_blocks, threads)
+        nrm64 = max(float(np.linalg.norm(fp64)), 1e-30)
         print(f"# exact-solve reference on {args.accuracy_blocks} blocks done", file=sys.stderr, flush=True)
         nrm = max(float(np.linalg.norm(exact)), 1e-30)
         var = {}
@@ -521,7 +541,8 @@ def main():
             v = work.d_speeds.cpu().numpy()[rows]
             var[name] = {"bm_ms": ms, "bm_section_steps_per_s": SECTIONS * st / (ms * 1e-3),
                          "jacobi_iters_per_step": sw / st, "passes_per_step": pa / st, "committed_steps": st,
-                         "rel_l2_vs_exact_solve": float(np.linalg.norm(v - exact) / nrm)}
+                         "rel_l2_vs_exact_solve": float(np.linalg.norm(v - exact) / nrm),
+                         "rel_l2_vs_fp64_rebuild": float(np.linalg.norm(v - fp64) / nrm64)}
             print(f"# variant {name}: {ms:.1f} ms, {sw / st:.2f} sweeps/step", file=sys.stderr, flush=True)
         out["variants"] = var
         lp, pl = var["learned_tol"], var["plain_tol"]
@@ -530,7 +551,8 @@ def main():
             "jacobi_iters_per_step_ratio": lp["jacobi_iters_per_step"] / pl["jacobi_iters_per_step"],
             "rel_l2_vs_exact_solve": {"learned": lp["rel_l2_vs_exact_solve"], "plain": pl["rel_l2_vs_exact_solve"]}}
         out["accuracy_sample"] = (f"BM velocity of {args.accuracy_blocks} sampled time blocks ({len(rows)} rows) "
-                                  "against the CPU oracle's exact per-pass pressure solve (FP32 state)")
+                                  "against the CPU oracle's exact per-pass pressure solve (FP32 state) and against "
+                                  "its FP64 rebuild of the reference method")
     if not args.no_cpu:
         # the oracle has no network: for the learned variant the CPU baseline is
         # the same solver with the extrapolated guess, checked bit-exact against
@@ -544,6 +566,10 @@ def main():
             gpu_speeds = work.d_speeds.cpu().numpy()
         cb, o, sel = cpu_baseline(work, args.cpu_blocks, threads, csc)
         cb["solver"] = cpu_variant
+        cb.update(host_cpu())
+        # one thread on a smaller sample: the per-core rate
+        cb1, _, _ = cpu_baseline(work, max(1, args.cpu_blocks // 16), 1, csc)
+        cb["single_thread"] = {"value": cb1["value"], "sample": cb1["sample"]}
         out["cpu_baseline"] = cb
         rows = block_rows(work, sel)
         d = np.linalg.norm(gpu_speeds[rows] - o["speeds"][rows]) / max(np.linalg.norm(o["speeds"][rows]), 1e-30)
