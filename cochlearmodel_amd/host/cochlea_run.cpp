@@ -59,6 +59,7 @@
 #include <fstream>
 #include <iostream>
 #include <map>
+#include <memory>
 #include <sstream>
 #include <stdexcept>
 #include <string>
@@ -475,13 +476,14 @@ struct JndLayout {
 JndLayout jnd_layout(const std::vector<float> &freqs, const std::vector<float> &levels,
                      const std::vector<float> &noises, bool one_ref_per_noise, double spl_ref,
                      const std::vector<double> &divisor, const std::vector<double> &fix_factor,
-                     const std::vector<double> &fix_add) {
+                     const std::vector<double> &fix_add, bool calculate_jnd = true) {
   constexpr float kNoNoise = 1111.f;   // MIN_INF_POWER_LEVEL
   auto spl2pa = [&](double db) { return db == kNoNoise ? 0.0 : 10.0 * std::pow(10.0, db / 20.0) * spl_ref; };
   const int tf = (int)freqs.size(), tp = (int)levels.size(), tn = (int)noises.size();
   if (tf == 0 || tp == 0 || tn == 0) throw std::runtime_error("testedFrequencies/PowerLevels/Noises must not be empty");
   JndLayout L;
-  const int add_f = one_ref_per_noise ? 0 : 1, add_n = 1 - add_f;
+  const int add_f = calculate_jnd && !one_ref_per_noise ? 1 : 0;
+  const int add_n = calculate_jnd && one_ref_per_noise ? 1 : 0;
   L.prof.assign((size_t)((tf + add_f) * tp + add_n) * tn, cm_jnd_profile{});
   int g = 0, calc = 0;
   auto reference = [&](int nl) {
@@ -509,7 +511,7 @@ JndLayout jnd_layout(const std::vector<float> &freqs, const std::vector<float> &
         const double pw = spl2pa(base);
         p.dBSPLSignal = base;
         p.dBSPLNoise = noises[nl] != kNoNoise ? noises[nl] : 0.f;
-        p.referenceIndex = one_ref_per_noise ? tp * tf * (nl + 1) + nl : g + tf - fl;
+        p.referenceIndex = !calculate_jnd ? -1 : one_ref_per_noise ? tp * tf * (nl + 1) + nl : g + tf - fl;
         p.calculateIndexPosition = calc++;
         p.dA = pw;
         p.recipdA = pw > 0 ? 1.0 / pw : 0.0;
@@ -517,13 +519,20 @@ JndLayout jnd_layout(const std::vector<float> &freqs, const std::vector<float> &
         p.frequency = freqs[fl];
         ++g;
       }
-      if (!one_ref_per_noise) reference(nl);
+      if (add_f) reference(nl);
     }
-    if (one_ref_per_noise) reference(nl);
+    if (add_n) reference(nl);
   }
   // JND_Serial_Intervals_Positions / JND_Interval_To_Reference (src/params.cpp:1596-1616)
   const int n = (int)L.prof.size(), nr = (int)L.refs.size();
   L.serial.assign(n, 0);
+  if (nr == 0) {   // no references: every interval is calculated in place
+    for (int i = 0; i < n; ++i) {
+      L.serial[i] = i;
+      L.calc.push_back(i);
+    }
+    return L;
+  }
   for (int i = 0; i < n; ++i) {
     const int rr = (int)((int64_t)i * nr / n);
     if (std::find(L.refs.begin(), L.refs.end(), i) == L.refs.end()) {
@@ -582,12 +591,19 @@ int run_jnd(const ParFile &par, const std::string &exe_dir) {
   const std::vector<float> noises = par.has("testedNoises") ? par.vec<float>("testedNoises")
                                                             : std::vector<float>{1111.f};
   const bool one_ref = par.scalar<int>("Generate_One_Ref_Per_Noise_Level", 1) == 1;
+  const bool calculate_jnd = par.scalar<int>("Calculate_JND", 0) == 1;
   const JndLayout lay = jnd_layout(freqs, levels, noises, one_ref, spl_ref,
                                    par.vec<double>("Complex_Profile_Power_Level_Divisor"),
                                    par.vec<double>("Complex_Profile_Noise_Level_Fix_Factor"),
-                                   par.vec<double>("Complex_Profile_Noise_Level_Fix_Addition"));
+                                   par.vec<double>("Complex_Profile_Noise_Level_Fix_Addition"), calculate_jnd);
   const std::string jnd_file = par.str("JND_File_Name");
-  if (jnd_file.empty()) throw std::runtime_error("JND File is missing, cannot output results...");
+  if (calculate_jnd && jnd_file.empty()) throw std::runtime_error("JND File is missing, cannot output results...");
+  const std::string out_path = par.str("Output_Result");
+  unsigned allowed = 0;
+  if (!par.scalar<int>("Discard_BM_Velocity_Output", 0)) allowed |= 1;
+  if (!par.scalar<int>("Discard_Lambdas_Output", 0)) allowed |= 14;
+  allowed = par.scalar<unsigned>("Allowed_Outputs", allowed);
+  if (!calculate_jnd && out_path.empty()) throw std::runtime_error("Output_Result is not set");
   const float dur = par.scalar<float>("JND_Interval_Duration", 0.02f);
   const float head_s = par.scalar<float>("JND_Interval_Head", 0.014f);
   const float tail_s = par.scalar<float>("JND_Interval_Tail", 0.002f);
@@ -649,7 +665,54 @@ int run_jnd(const ParFile &par, const std::string &exe_dir) {
   const float bm_ms = cm_last_kernel_ms();
   std::vector<int32_t> failed_blocks(TB);
   ok(GeneralKernel_Copy_Results_Bytes(failed_blocks.data(), getCudaFailedBlocks(), TB * 4, 0), "failed blocks");
+  // Output_Result of a profile run (src/solver.cpp:704-734): decoupled
+  // intervals show the transient and drop the tail (Remove_Transient_Tail), so
+  // rows [0, rows) are written
+  std::unique_ptr<BinWriter> out;
+  const int conv_tests = par.scalar<int>("Convergence_Tests", 0);
+  if (!out_path.empty()) {
+    out.reset(new BinWriter(out_path));
+    auto dev_copy = [&](void *src, size_t n_bytes, void *dst) {
+      ok(GeneralKernel_Copy_Results_Bytes(dst, src, n_bytes, 0), "GeneralKernel_Copy_Results_Bytes");
+    };
+    std::vector<int32_t> failed_nodes(bufP1);
+    dev_copy(getCudaFailedTimeNodes(), failed_nodes.size() * 4, failed_nodes.data());
+    std::vector<float> conv_times(bufP1), conv_times_b(TB), conv_loops(bufP1), conv_loops_b(TB), conv_j(bufP1),
+        conv_j_b(TB);
+    ok(extractConvergenceTimes(conv_times.data(), conv_times.size()), "extractConvergenceTimes");
+    dev_copy(getCudaConvergeSpeedBlocks(), TB * 4, conv_times_b.data());
+    dev_copy(getCudaConvergedTimeNodes(), bufP1 * 4, conv_loops.data());
+    dev_copy(getCudaConvergedBlocks(), TB * 4, conv_loops_b.data());
+    dev_copy(getCudaConvergedJacobyLoopsPerIteration(), bufP1 * 4, conv_j.data());
+    dev_copy(getCudaConvergedJacobyLoopsPerIterationBlocks(), TB * 4, conv_j_b.data());
+    if (conv_tests & 1) out->write(conv_times.data(), rows);
+    if (conv_tests & 2) out->write(conv_times_b.data(), TB);
+    out->write_cast(failed_nodes.data(), rows);
+    out->write_cast(failed_blocks.data(), TB);
+    if (conv_tests & 4) {
+      out->write(conv_loops.data(), rows);
+      out->write(conv_loops_b.data(), TB);
+    }
+    if (conv_tests & 8) {
+      out->write(conv_j.data(), rows);
+      out->write(conv_j_b.data(), TB);
+    }
+    if (allowed & 1) {
+      std::vector<float> speeds((size_t)rows * kSections);
+      ok(BMOHCKernel_Copy_Results(speeds.data(), speeds.size(), 0), "BMOHCKernel_Copy_Results");
+      out->write(speeds.data(), speeds.size());
+    }
+  }
+  const bool need_lambda = calculate_jnd || (out && (allowed & 14));
 
+  if (!need_lambda) {
+    const float timers[2] = {bm_ms, 0.f};
+    if (out) out->write(timers, 2);
+    std::printf("cochlea_run: profile run, %d intervals written to %s (BM %.2f ms)\n", n_prof, out_path.c_str(),
+                bm_ms);
+    ok(BMOHCKernel_Free(), "BMOHCKernel_Free");
+    return 0;
+  }
   // IHC / lambda over the decoupled intervals
   std::vector<double> b, a;
   const bool fir = ac_filter(par, exe_dir, Fs, b, a);
@@ -672,7 +735,27 @@ int run_jnd(const ParFile &par, const std::string &exe_dir) {
   for (int j = 0; j < kLambdas; ++j)
     for (int i = 0; i < kSections; ++i) aihc[j * kSections + i] = A3[j];
   ok(loadAihc(aihc.data()), "loadAihc");
+  const auto t_ihc = std::chrono::steady_clock::now();
   ok(RunIHCKernel(nullptr, 0, 0, 0, 0), "RunIHCKernel");
+  ok(cm_device_synchronize(), "cm_device_synchronize");
+  const float ihc_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t_ihc).count();
+  if (out) {
+    if (allowed & 14) {
+      std::vector<float> lam((size_t)kLambdas * rows * kSections);
+      ok(BMOHCKernel_Copy_Lambda(lam.data(), lam.size(), 0), "BMOHCKernel_Copy_Lambda");
+      for (int j = 0; j < kLambdas; ++j)
+        if (allowed & (2u << j)) out->write(lam.data() + (size_t)j * rows * kSections, (size_t)rows * kSections);
+    }
+    const float timers[2] = {bm_ms, ihc_ms};
+    out->write(timers, 2);
+  }
+  if (!calculate_jnd) {
+    std::printf("cochlea_run: profile run, %d intervals written to %s (BM %.2f ms)\n", n_prof, out_path.c_str(),
+                bm_ms);
+    ok(BMOHCKernel_Free(), "BMOHCKernel_Free");
+    ok(IHCKernel_Free(), "IHCKernel_Free");
+    return 0;
+  }
 
   // JND reductions, final dB, failed signals
   std::vector<double> nIHC(kSections, 30000.0 / kSections);
@@ -749,7 +832,8 @@ int main(int argc, char **argv) {
       const std::vector<float> noises = par.has("testedNoises") ? par.vec<float>("testedNoises")
                                                                 : std::vector<float>{1111.f};
       const JndLayout lay = jnd_layout(freqs, levels, noises, par.scalar<int>("Generate_One_Ref_Per_Noise_Level", 1) == 1,
-                                       par.scalar<double>("SPLRef", 2e-5), {}, {}, {});
+                                       par.scalar<double>("SPLRef", 2e-5), {}, {}, {},
+                                       par.scalar<int>("Calculate_JND", 1) == 1);
       for (size_t g = 0; g < lay.prof.size(); ++g) {
         const cm_jnd_profile &q = lay.prof[g];
         std::printf("%zu %.17g %.17g %.17g %d %d %d %d\n", g, q.dA, q.Wn, q.frequency, (int)q.isReference,
@@ -763,7 +847,7 @@ int main(int argc, char **argv) {
     }
     ParFile par(argv[1]);
     const int sim_type = par.scalar<int>("Sim_Type", 0);
-    if ((sim_type == 3 || sim_type == 4) && par.scalar<int>("Calculate_JND", 0) == 1)
+    if (sim_type == 3 || (sim_type == 4 && par.scalar<int>("Calculate_JND", 0) == 1))
       return run_jnd(par, dir_of(argv[0]));
     return run(par, dir_of(argv[0]));
   } catch (const std::exception &e) {

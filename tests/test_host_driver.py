@@ -232,11 +232,12 @@ def test_complex_jnd_run_files(tmp_path):
     res = sw.finish()
     wn = tmp_path / "noise.f64"
     sw.WN.astype(np.float64).tofile(wn)
-    fin, raw = tmp_path / "jnd_final.bin", tmp_path / "jnd_raw.bin"
+    fin, raw, res_bin = tmp_path / "jnd_final.bin", tmp_path / "jnd_raw.bin", tmp_path / "jnd_out.bin"
+    common = ("Fs = 44100\ntestedFrequencies = 1000 3000\ntestedPowerLevels = 20 40 60\ntestedNoises = 1111 40\n"
+              f"JND_Interval_Duration = 0.04\nInput_Noise_File = {wn}\nAllowed_Outputs = 15\n")
     (tmp_path / "jnd.par").write_text(
-        "Sim_Type = 4\nCalculate_JND = 1\nFs = 44100\ntestedFrequencies = 1000 3000\ntestedPowerLevels = 20 40 60\n"
-        f"testedNoises = 1111 40\nJND_Interval_Duration = 0.04\nComplex_JND_Calculation_Types = 3\n"
-        f"Input_Noise_File = {wn}\nJND_File_Name = {fin}\nJND_Raw_File_Name = {raw}\n")
+        "Sim_Type = 4\nCalculate_JND = 1\nComplex_JND_Calculation_Types = 3\n" + common +
+        f"JND_File_Name = {fin}\nJND_Raw_File_Name = {raw}\nOutput_Result = {res_bin}\n")
     r = subprocess.run([EXE, str(tmp_path / "jnd.par")], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
     n = len(sw.calc)
@@ -259,6 +260,44 @@ def test_complex_jnd_run_files(tmp_path):
     np.testing.assert_allclose(got_fin[:val.size], val.reshape(-1), atol=1e-3)
     np.testing.assert_array_equal(got_fin[val.size:2 * val.size], warn.reshape(-1).astype(np.float32))
     np.testing.assert_array_equal(got_fin[2 * val.size:], conv.reshape(-1).astype(np.float32))
+    # Output_Result of the profile run: failed nodes and blocks, velocity and the
+    # three lambda planes over rows [0, intervals x interval), then two timers
+    rows = sw.rows
+    stream = np.fromfile(res_bin, np.float32)
+    assert stream.size == rows + sw.TB + 4 * rows * 256 + 2
+    np.testing.assert_array_equal(stream[rows:rows + sw.TB], sw.d_failed_blocks.cpu().numpy().astype(np.float32))
+    vel = stream[rows + sw.TB:rows + sw.TB + rows * 256].reshape(rows, 256)
+    np.testing.assert_array_equal(vel, sw.d_speeds[:rows].cpu().numpy())
+    lam = stream[rows + sw.TB + rows * 256:-2].reshape(3, rows, 256)
+    np.testing.assert_array_equal(lam, sw.d_lambda.cpu().numpy())
+    # Sim_Type 3 without JND: the same tones without reference intervals; each
+    # interval's velocity equals that of the same profile in the run above
+    res3 = tmp_path / "prof_out.bin"
+    (tmp_path / "prof.par").write_text("Sim_Type = 3\nCalculate_JND = 0\n" + common + f"Output_Result = {res3}\n")
+    r3 = subprocess.run([EXE, str(tmp_path / "prof.par")], capture_output=True, text=True, timeout=300)
+    assert r3.returncode == 0, r3.stdout + r3.stderr
+    n3 = len(sw.calc)
+    rows3 = n3 * sw.interval
+    tb3 = n3 * sw.DF
+    s3 = np.fromfile(res3, np.float32)
+    assert s3.size == rows3 + tb3 + 4 * rows3 * 256 + 2
+    vel3 = s3[rows3 + tb3:rows3 + tb3 + rows3 * 256].reshape(n3, sw.interval, 256)
+    vel_all = vel.reshape(sw.n_prof, sw.interval, 256)
+    for p_i, j in enumerate(sw.calc):
+        np.testing.assert_array_equal(vel3[p_i], vel_all[j], err_msg=f"profile {p_i}")
+
+
+def test_profile_layout_without_jnd(tmp_path):
+    """Calculate_JND = 0: no reference intervals, reference indexes -1."""
+    p = tmp_path / "p.par"
+    p.write_text("testedFrequencies = 1000 3000\ntestedPowerLevels = 20 40\ntestedNoises = 1111 40\n"
+                 "Calculate_JND = 0\n")
+    r = subprocess.run([EXE, "--profiles", str(p)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    rows = [line.split() for line in r.stdout.splitlines()]
+    assert len(rows) == 8
+    assert all(int(x[4]) == 0 and int(x[6]) == -1 for x in rows)
+    assert [int(x[5]) for x in rows] == list(range(8))
 
 
 @pytest.mark.parametrize("one_ref", [1, 0])
