@@ -121,21 +121,25 @@ def test_exact_targets_match_dense_solve():
     assert np.linalg.norm(T - ref) <= 1e-5 * np.linalg.norm(ref)
 
 
-def test_in_solver_network_matches_oracle():
+@pytest.mark.parametrize("n,dur,per_item,group", [(4, 0.04, 64, 8), (2200, 0.02, 4, 10), (2700, 0.02, 4, 12)])
+def test_in_solver_network_matches_oracle(n, dur, per_item, group):
     """The MFMA inference inside the solver (fragment packing, tile mapping,
-    GELU, bf16 operands) against the numpy oracle on the captured inputs."""
+    GELU, bf16 operands) against the numpy oracle on the captured inputs, for
+    each lockstep group size (8, 10, 12 blocks: the launcher picks the smallest
+    whose groups fit one per CU of 256)."""
     _cuda()
     from cochlearmodel_amd import warmstart
     from cochlearmodel_amd.mlp import SIZES
-    _say("case: random network inside the solver, outputs captured")
+    _say(f"case: random network inside the solver, outputs captured, {n} blocks (group {group})")
     rng = np.random.default_rng(11)
     params = mo.init_params(SIZES, seed=3)
     params[1] = (0.1 * rng.standard_normal(SIZES[1])).astype(np.float32)
     params[3] = (0.1 * rng.standard_normal(SIZES[2])).astype(np.float32)
     params[5] = (0.1 * rng.standard_normal(SIZES[3])).astype(np.float32)
     warmstart.upload(params, 1.0)
-    tb = _batch(n=4, dur=0.04, seed=8)
-    _, _, rec = _capture_run(tb, warmstart.tol_scalars(1e-6, 2), 131, 64)
+    tb = _batch(n=n, dur=dur, seed=8)
+    assert tb.TB == n * int(round(dur / 0.02))
+    _, _, rec = _capture_run(tb, warmstart.tol_scalars(1e-6, 2), 131, per_item)
     R = rec[rec[:, 770] > 0]
     assert len(R) > 50
     y = R[:, 1288:1544].astype(np.float64)
