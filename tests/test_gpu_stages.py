@@ -315,3 +315,54 @@ def test_diagnostic_entry_points():
     assert ms >= 0.0
     _lib.check(L.cudaEventsDestroy(a, b))
     assert L.cudaEventsStopQueryTimer(None, None, 0, b"off") == 0.0
+
+
+def test_hearing_aid_transfer_functions(oracle):
+    """generateInputFromProfile's hearing-aid stage (src/cochlea.cu:4104-4150):
+    per profile interval an FIR over min(pos, taps) past samples (position 0
+    gets 0), then the 'IIR' stage, which adds taps 1.. of its vector over the
+    FIR output.  A pure one-sample delay pins the window rule; general taps are
+    checked against a float64 restatement."""
+    _cuda()
+    from cochlearmodel_amd import _lib, solver
+    from cochlearmodel_amd._lib import check, ptr
+    _say("case: hearing-aid FIR/IIR on generated profiles")
+    c = _c4_small()
+    L = _lib.lib()
+    sec = solver.section_params()
+    g = np.full(256, 0.5, np.float32)
+    keep = [np.ascontiguousarray(x, np.float32) for x in (g, sec[0], sec[1], g, g, sec[2], sec[3], sec[4], sec[5],
+                                                          sec[6], sec[7])]
+    check(L.BMOHCKernel_Init(*[ptr(k) for k in keep], 1.0, 0.0, c["bufP1"] + 10, (c["bufP1"] + 10) * 256,
+                             0, 1, 0, 0), "init")
+    check(L.InputProfilesArrayInitialization(c["n_prof"], c["L"], 0, 0, 0), "init")
+
+    def gen(fir, iir):
+        fir = np.ascontiguousarray(fir, np.float32)
+        iir = np.ascontiguousarray(iir, np.float32)
+        out = np.zeros(c["n_prof"] * c["L"], np.float32)
+        check(L.generateInputFromProfile(ptr(c["prof"]), ptr(c["WN"]), c["L"], None, 0, 0, 0, 0, 0.16, 0.0, 0,
+                                         c["e1"], 0, c["e1"], c["n_prof"], 0, 0, c["n_prof"], c["ov"], c["L"],
+                                         c["head"], c["length"], FS, 1, ptr(out), 0, 1, 1e-15, 1e-10, 0, 0.0, 0.0,
+                                         c["DF"], 0, ptr(fir), len(fir), ptr(iir), len(iir)), "generate")
+        return out.reshape(c["n_prof"], c["L"])
+
+    x0 = gen([1.0], [1.0])
+    delay = gen([0.0, 1.0], [1.0])
+    np.testing.assert_array_equal(delay[:, 2:], x0[:, 1:-1])
+    np.testing.assert_array_equal(delay[:, :2], 0.0)
+    h = np.array([0.5, 0.3, -0.2, 0.1], np.float32)
+    gi = np.array([1.0, -0.25, 0.125], np.float32)
+    got = gen(h, gi)
+    x = x0.astype(np.float64)
+    tmp = np.zeros_like(x)
+    for pos in range(x.shape[1]):
+        for i in range(min(pos, len(h))):
+            tmp[:, pos] += float(h[i]) * x[:, pos - i]
+    ref = tmp.copy()
+    for pos in range(x.shape[1]):
+        for j in range(1, min(pos + 1, len(gi))):
+            ref[:, pos] += float(gi[j]) * tmp[:, pos - j]
+    rel = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    _say(f"hearing aid vs float64 restatement: rel L2 {rel:.2e}")
+    assert rel <= 1e-6
