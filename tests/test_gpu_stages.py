@@ -323,7 +323,6 @@ def test_hearing_aid_transfer_functions(oracle):
     gets 0), then the 'IIR' stage, which adds taps 1.. of its vector over the
     FIR output.  A pure one-sample delay pins the window rule; general taps are
     checked against a float64 restatement."""
-    _cuda()
     from cochlearmodel_amd import _lib, solver
     from cochlearmodel_amd._lib import check, ptr
     _say("case: hearing-aid FIR/IIR on generated profiles")
