@@ -105,14 +105,16 @@ __device__ __forceinline__ float in_at(const BmLaunch &a, int64_t idx) {
 
 // J Jacobi sweeps of the kernel convention (include/cochlea.cuh:13962-13982):
 // p_i <- (Y_i - (l_i p_{i-1} + p_{i+1})) / m_i with the lane-boundary
-// neighbours exchanged by DPP wave shifts (zero past either end).
+// neighbours exchanged by DPP wave shifts (zero past either end).  l_i is 0 or
+// 1, so l_i p_{i-1} is exact and fma(l_i, p_{i-1}, p_{i+1}) rounds exactly as
+// the separate multiply and add do: one instruction fewer per sweep.
 __device__ __forceinline__ void jacobi_sweep(const float (&Y)[4], const float (&rsm)[4], float low23,
                                              const float (&p)[4], float (&q)[4]) {
   const float left = wave_shr1(p[3]);
   const float right = wave_shl1(p[0]);
   q[0] = (Y[0] - (left + p[1])) * rsm[0];
   q[1] = (Y[1] - (p[0] + p[2])) * rsm[1];
-  q[2] = (Y[2] - (low23 * p[1] + p[3])) * rsm[2];
+  q[2] = (Y[2] - __builtin_fmaf(low23, p[1], p[3])) * rsm[2];
   q[3] = (Y[3] - (low23 * p[2] + right)) * rsm[3];
 }
 // J is wave-uniform: the count goes to a scalar register, and sweeps are
@@ -141,7 +143,7 @@ __device__ __forceinline__ void resid_vec(const float (&Y)[4], const float (&mf)
   const float right = wave_shl1(p[0]);
   r[0] = Y[0] - ((left + p[1]) + mf[0] * p[0]);
   r[1] = Y[1] - ((p[0] + p[2]) + mf[1] * p[1]);
-  r[2] = Y[2] - ((low23 * p[1] + p[3]) + mf[2] * p[2]);
+  r[2] = Y[2] - (__builtin_fmaf(low23, p[1], p[3]) + mf[2] * p[2]);
   r[3] = Y[3] - ((low23 * p[2] + right) + mf[3] * p[3]);
 }
 __device__ __forceinline__ float resid2(const float (&Y)[4], const float (&mf)[4], float low23, const float (&p)[4]) {
@@ -502,11 +504,14 @@ __device__ void run_item(const BmLaunch &a, int item_index, int lane, float4 *po
 template <int WAVES, bool TOL, int WS>
 __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(3))) void bm_kernel(BmLaunch a) {
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
+  // wave-uniform by construction: readfirstlane lets the compiler keep the
+  // item's fields and the whole step control in scalar registers
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int idx = blockIdx.x * WAVES + wave;
   __shared__ float4 pold_lds[WS ? WAVES * 65 : 1];
   if (idx < a.n_items)
-    run_item<TOL, WS>(a, a.order ? a.order[idx] : idx, lane, pold_lds + (WS ? wave * 65 : 0));
+    run_item<TOL, WS>(a, __builtin_amdgcn_readfirstlane(a.order ? a.order[idx] : idx), lane,
+                      pold_lds + (WS ? wave * 65 : 0));
 }
 
 // ------------------------------------------- learned warm start ---------
