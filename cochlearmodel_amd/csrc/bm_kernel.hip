@@ -77,14 +77,25 @@ __device__ __forceinline__ float readlane(float x, int l) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
 }
 
-// Sum of 256 values, 4 per lane, with the fixed tree.
+// Row broadcasts: every row r >= 1 <- lane 15 of row r-1; rows 2 and 3 <-
+// lane 31.  Only row 3 is read afterwards, so what the other rows receive
+// does not matter and the full row mask lets the move fold into the add.
+__device__ __forceinline__ float row_bcast15(float x) { return CM_DPP(x, 0x142); }
+__device__ __forceinline__ float row_bcast31(float x) { return CM_DPP(x, 0x143); }
+
+// Sum of 256 values, 4 per lane, with the fixed tree
+// ((r0 + r1) + (r2 + r3)) of the row sums r_k.  After the xor steps every lane
+// of row k holds r_k; the two row broadcasts leave (r3 + r2) + (r1 + r0) in
+// row 3, which equals the tree because each addition is commutative.
 __device__ __forceinline__ float tree_sum(float x0, float x1, float x2, float x3) {
   float s = (x0 + x1) + (x2 + x3);
   s = s + quad_xor1(s);
   s = s + quad_xor2(s);
   s = s + row_half_mirror(s);   // equals xor 4 once quads are uniform
   s = s + row_mirror(s);        // equals xor 8 once octets are uniform
-  return (readlane(s, 0) + readlane(s, 16)) + (readlane(s, 32) + readlane(s, 48));
+  s = s + row_bcast15(s);
+  s = s + row_bcast31(s);
+  return readlane(s, 63);
 }
 __device__ __forceinline__ float tree_max(float x0, float x1, float x2, float x3) {
   float s = fmaxf(fmaxf(x0, x1), fmaxf(x2, x3));
@@ -92,7 +103,9 @@ __device__ __forceinline__ float tree_max(float x0, float x1, float x2, float x3
   s = fmaxf(s, quad_xor2(s));
   s = fmaxf(s, row_half_mirror(s));
   s = fmaxf(s, row_mirror(s));
-  return fmaxf(fmaxf(readlane(s, 0), readlane(s, 16)), fmaxf(readlane(s, 32), readlane(s, 48)));
+  s = fmaxf(s, row_bcast15(s));
+  s = fmaxf(s, row_bcast31(s));
+  return readlane(s, 63);
 }
 
 
@@ -306,17 +319,18 @@ __device__ __forceinline__ float pass_rhs(const ItemConst &k, const ItemScal &c,
 }
 
 // Jacobi until |Y - A p|_2 <= tol |Y|_2 (checked every check_every sweeps,
-// capped at max_sweeps); returns the sweeps done.
+// capped at max_sweeps); returns the sweeps done.  rr is |Y - A p|_2^2 of the
+// incoming p (the guess selection has already computed it).
 __device__ __forceinline__ int jacobi_tol(const cm_bm_scalars &sc, const ItemConst &k, const float (&Y)[4],
-                                          float yy, float (&p)[4]) {
+                                          float yy, float (&p)[4], float rr) {
   const float tau2 = sc.tol_rel * sc.tol_rel;
   int J = 0;
   for (;;) {
-    const float rr = resid2(Y, k.mf, k.low23, p);
     if (rr <= tau2 * yy || J >= sc.max_sweeps) break;
     const int kk = sc.check_every < sc.max_sweeps - J ? sc.check_every : sc.max_sweeps - J;
     jacobi_sweeps(Y, k.rsm, k.low23, p, kk);
     J += kk;
+    rr = resid2(Y, k.mf, k.low23, p);
   }
   return J;
 }
@@ -483,11 +497,15 @@ __device__ void run_item(const BmLaunch &a, int item_index, int lane, float4 *po
       int J;
       if (TOL) {
         const float yy = tree_sum(Y[0] * Y[0], Y[1] * Y[1], Y[2] * Y[2], Y[3] * Y[3]);
+        float rr;
         if (WS == 1 && out_loop == 0) {
           float rre;
-          (void)extrapolate_guess(k, Y, s.dt, pold, lane, s.p, rre, false);
+          const float rr0 = extrapolate_guess(k, Y, s.dt, pold, lane, s.p, rre, false);
+          rr = rre < rr0 ? rre : rr0;   // residual of the guess kept
+        } else {
+          rr = resid2(Y, k.mf, k.low23, s.p);
         }
-        J = jacobi_tol(a.sc, k, Y, yy, s.p);
+        J = jacobi_tol(a.sc, k, Y, yy, s.p, rr);
       } else {
         J = out_loop < 2 ? a.sc.j_slow : a.sc.j_fast;
         jacobi_sweeps(Y, k.rsm, k.low23, s.p, J);
@@ -723,6 +741,7 @@ __global__ __launch_bounds__(64 * G) void bm_ws_kernel(BmLaunch a, WsArgs w) {
       pass_begin(s, true);
       const float sf = pass_rhs(k, c, s, true, lane, od, G4, Y);
       const float yy = tree_sum(Y[0] * Y[0], Y[1] * Y[1], Y[2] * Y[2], Y[3] * Y[3]);
+      float rr_sel = rre;   // residual of the guess kept (p_lin unless replaced below)
       if (net_round) {
         // learned guess: p_lin + rms(r_lin) c_out S(y), S = [1,2,1]/4 (edges replicated)
         const float4 y4 = *reinterpret_cast<const float4 *>(reinterpret_cast<const float *>(H2s) + wave * WS_OS +
@@ -742,18 +761,21 @@ __global__ __launch_bounds__(64 * G) void bm_ws_kernel(BmLaunch a, WsArgs w) {
         const float rrh = resid2(Y, k.mf, k.low23, ph);
         if (rrh < rre && rrh < rr0) {
           UNROLL for (int j = 0; j < 4; ++j) s.p[j] = ph[j];
+          rr_sel = rrh;
         } else if (!(rre < rr0)) {
           const float4 pk = pold[lane];
           s.p[0] = pk.x; s.p[1] = pk.y; s.p[2] = pk.z; s.p[3] = pk.w;
+          rr_sel = rr0;
         }
       } else if (!(rre < rr0)) {
         const float4 pk = pold[lane];
         s.p[0] = pk.x; s.p[1] = pk.y; s.p[2] = pk.z; s.p[3] = pk.w;
+        rr_sel = rr0;
       }
       // rest of the predictor pass, then the corrector passes of this step
       int out_loop = 0, reset = 1, another = 1;
       {
-        const int J = jacobi_tol(a.sc, k, Y, yy, s.p);
+        const int J = jacobi_tol(a.sc, k, Y, yy, s.p, rr_sel);
         s.n_sweeps += J;
         pass_finish(k, c, s, od, G4, sf);
         float vref[4], aref[4];   // unused: step control is skipped on the predictor pass
@@ -770,7 +792,7 @@ __global__ __launch_bounds__(64 * G) void bm_ws_kernel(BmLaunch a, WsArgs w) {
         float od2[4], G2[4], Y2[4];
         const float sf2 = pass_rhs(k, c, s, out_loop < 2, lane, od2, G2, Y2);
         const float yy2 = tree_sum(Y2[0] * Y2[0], Y2[1] * Y2[1], Y2[2] * Y2[2], Y2[3] * Y2[3]);
-        const int J = jacobi_tol(a.sc, k, Y2, yy2, s.p);
+        const int J = jacobi_tol(a.sc, k, Y2, yy2, s.p, resid2(Y2, k.mf, k.low23, s.p));
         s.n_sweeps += J;
         pass_finish(k, c, s, od2, G2, sf2);
         pass_control(a.sc, c, s, vref, aref, out_loop, reset, another);
