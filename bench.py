@@ -80,7 +80,8 @@ PEAK_BF16_TFLOPS = 2500.0    # MI355X dense BF16 MFMA
 # profiles/pmc_valu_c2_reference.md, profiles/pmc_mix_c2_learned.md) and the issue
 # capacity they are measured against: a wave64 FP32 VALU instruction holds a SIMD
 # for 4 cycles, 1024 SIMDs at 2.4 GHz
-MEASURED_VALU_INSTS = {"reference_20_2": 1.0995e11, "learned_tol": 2.7545e11}
+MEASURED_VALU_INSTS = {"reference_20_2": 1.0995e11, "learned_tol": 2.7545e11,
+                       "ihc_lambda": 1.40262e9 + 2.68728e9}   # ac_ds_kernel + dc_lambda_kernel
 VALU_ISSUE_PER_S = 1024 * 2.4e9 / 4
 # L1<-L2 reads of the learned kernel: TCP_TCC_READ_REQ_sum x 128 B (a 1 GiB copy
 # reads 8.389e6 requests); profiles/pmc_mix_c2_learned.md
@@ -507,6 +508,12 @@ def main():
                                 "traffic": MEASURED_HBM_BYTES["ihc_lambda"] if args.signals == 256 else None,
                                 "fp64_flop_per_launch": work.rows * SECTIONS * 2.0 * (76 + 88)},
     }
+    if args.signals == 256:
+        # the IHC/lambda FIRs run in FP64: besides HBM, the stage is bound by VALU issue
+        out["ihc_lambda_roofline"]["valu_issue"] = {
+            "insts_per_launch": MEASURED_VALU_INSTS["ihc_lambda"],
+            "frac": MEASURED_VALU_INSTS["ihc_lambda"] / (float(np.mean(ihc_ms)) * 1e-3) / VALU_ISSUE_PER_S,
+            "capacity_per_s": VALU_ISSUE_PER_S}
     if args.signals == 256 and args.variant in MEASURED_VALU_INSTS:
         out["roofline"]["valu_issue"] = {
             "insts_per_launch": MEASURED_VALU_INSTS[args.variant],
