@@ -337,7 +337,7 @@ def c4_main(args):
     flops = flops_per_launch(steps, passes, sweeps)
     bm_tf = flops / (avg["bm_ms"] * 1e-3) / 1e12
     lam_bytes = sw.rows * SECTIONS * 16.0
-    jnd_bytes = sw.n_prof * sw.length * SECTIONS * 24.0       # two passes over 3 lambda planes
+    jnd_bytes = sw.n_prof * sw.length * SECTIONS * 24.0       # 3 lambda planes of each body and of its reference
     out = {
         "metric": "section-steps/s (JND sweep)", "value": SECTIONS * steps / (ms_per_step * 1e-3),
         "unit": "section-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

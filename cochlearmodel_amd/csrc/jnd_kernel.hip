@@ -1,6 +1,6 @@
-// JND reductions of CudaCalculateJND (src/cochlea.cu:3703-4065):
-// per-interval mean rate (GlobalCalculateMeanRate :3741-3771) and the Fisher
-// AI / rate terms reduced over the 256 sections (GlobalCalculateJND
+// JND reductions of CudaCalculateJND (src/cochlea.cu:3703-4065) in one
+// kernel: per-interval mean rate (GlobalCalculateMeanRate :3741-3771) and the
+// Fisher AI / rate terms reduced over the 256 sections (GlobalCalculateJND
 // :3773-3944).  One workgroup = 256 sections of one (interval, fiber group);
 // time sums run down the section-fastest lambda planes, so every step is a
 // coalesced 1 KB read.  The section reduction keeps the reference's halving
@@ -12,14 +12,6 @@
 namespace cm {
 
 
-
-__global__ __launch_bounds__(256) void mean_rate_kernel(JndArgs a) {
-  const int iv = blockIdx.x, li = blockIdx.y, s = threadIdx.x;
-  const int64_t off = ((int64_t)li * a.rows + (int64_t)iv * a.interval_nodes + a.head_nodes) * CM_SECTIONS + s;
-  double acc = a.lambda[off];
-  for (int t = 1; t < a.length_nodes; ++t) acc = acc + a.lambda[off + (int64_t)t * CM_SECTIONS];
-  a.mean_rate[((int64_t)li * a.n_intervals + iv) * CM_SECTIONS + s] = (float)acc / (float)a.length_nodes;
-}
 
 __device__ __forceinline__ float halving_tree(float *sh, float v, int s) {
   sh[s] = v;
@@ -33,32 +25,57 @@ __device__ __forceinline__ float halving_tree(float *sh, float v, int s) {
   return r;
 }
 
+// One pass over the interval body of (interval, fiber group) and of its
+// reference: both mean rates (FP64 sums in time order, as
+// GlobalCalculateMeanRate takes them) and the AI Fisher sum, then the rate
+// term and the two section reductions.  The reference's mean rate is summed
+// again by every interval that uses it, in the same order, so it is the value
+// its own workgroup writes; fusing the mean-rate pass saves one read of every
+// interval body.
 __global__ __launch_bounds__(256) void jnd_kernel(JndArgs a) {
   __shared__ float sh[CM_SECTIONS];
   const int iv = blockIdx.x, li = blockIdx.y, s = threadIdx.x;
   const int ref = a.ref_index[iv];
   const float dAv = a.dA[iv];
-  const float mr = a.mean_rate[((int64_t)li * a.n_intervals + iv) * CM_SECTIONS + s];
-  const float mref = a.mean_rate[((int64_t)li * a.n_intervals + ref) * CM_SECTIONS + s];
-  const float dMR = (mr - mref) / dAv;
+  const int64_t c = ((int64_t)li * a.rows + (int64_t)iv * a.interval_nodes + a.head_nodes) * CM_SECTIONS + s;
+  const int64_t r = ((int64_t)li * a.rows + (int64_t)ref * a.interval_nodes + a.head_nodes) * CM_SECTIONS + s;
+  const int L = a.length_nodes;
   float ai = 0.f, rms = 0.f;
-  if (a.calc_ai) {
-    const int64_t c = ((int64_t)li * a.rows + (int64_t)iv * a.interval_nodes + a.head_nodes) * CM_SECTIONS + s;
-    const int64_t r = ((int64_t)li * a.rows + (int64_t)ref * a.interval_nodes + a.head_nodes) * CM_SECTIONS + s;
+  if (a.calc_ai || a.calc_rms) {
+    double acc_c = a.lambda[c], acc_r = a.lambda[r];
     float accv = 0.0f;
-    for (int t = 0; t < a.length_nodes; ++t) {
-      const float refl = a.lambda[r + (int64_t)t * CM_SECTIONS];
-      const float cal = a.lambda[c + (int64_t)t * CM_SECTIONS];
+    if (a.calc_ai) {
+      const float refl = a.lambda[r], cal = a.lambda[c];
       const float dl = dAv > 0 ? (cal - refl) / dAv : 0.0f;
-      const float pre = (float)((double)(dl * dl) / (a.Fs * (double)refl));
-      accv += pre;
+      accv += (float)((double)(dl * dl) / (a.Fs * (double)refl));
+#pragma unroll 4
+      for (int t = 1; t < L; ++t) {
+        const float refl = a.lambda[r + (int64_t)t * CM_SECTIONS];
+        const float cal = a.lambda[c + (int64_t)t * CM_SECTIONS];
+        acc_c = acc_c + cal;
+        acc_r = acc_r + refl;
+        const float dl = dAv > 0 ? (cal - refl) / dAv : 0.0f;
+        accv += (float)((double)(dl * dl) / (a.Fs * (double)refl));
+      }
+    } else {
+#pragma unroll 8
+      for (int t = 1; t < L; ++t) {
+        acc_c = acc_c + a.lambda[c + (int64_t)t * CM_SECTIONS];
+        acc_r = acc_r + a.lambda[r + (int64_t)t * CM_SECTIONS];
+      }
     }
-    const float rs = 1.0f / __builtin_sqrtf(accv);
-    ai = (float)((double)(a.T / a.Tlength) * a.nIHC[s] / (double)(rs * rs));
-  }
-  if (a.calc_rms) {
-    const float crlb = 1.0f / __builtin_sqrtf(((a.T / mref) * dMR) * dMR);
-    rms = (float)(a.nIHC[s] / (double)(crlb * crlb));
+    const float mr = (float)acc_c / (float)L;
+    const float mref = (float)acc_r / (float)L;
+    a.mean_rate[((int64_t)li * a.n_intervals + iv) * CM_SECTIONS + s] = mr;
+    if (a.calc_ai) {
+      const float rs = 1.0f / __builtin_sqrtf(accv);
+      ai = (float)((double)(a.T / a.Tlength) * a.nIHC[s] / (double)(rs * rs));
+    }
+    if (a.calc_rms) {
+      const float dMR = (mr - mref) / dAv;
+      const float crlb = 1.0f / __builtin_sqrtf(((a.T / mref) * dMR) * dMR);
+      rms = (float)(a.nIHC[s] / (double)(crlb * crlb));
+    }
   }
   const int64_t fi = (int64_t)li * a.out_stride + a.out_index[iv];
   if (a.calc_ai) {
@@ -93,8 +110,6 @@ extern "C" int cm_launch_jnd(const cm::JndArgs *a, hipStream_t stream) {
   using namespace cm;
   if (a->n_intervals <= 0) return 0;
   if (a->n_intervals > 0x7fffffff) return -3;
-  if (a->calc_rms || a->calc_ai)
-    hipLaunchKernelGGL(mean_rate_kernel, dim3(a->n_intervals, 3), dim3(256), 0, stream, *a);
   hipLaunchKernelGGL(jnd_kernel, dim3(a->n_intervals, 3), dim3(256), 0, stream, *a);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
