@@ -832,19 +832,21 @@ extern "C" int cm_launch_bm(const cm::BmLaunch *a, int waves_per_block, int grid
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
-// Lockstep groups of G blocks: the smallest G in {8, 10, 12} whose groups fit
-// one per CU, else 12.  Returns G (> 0) or a negative error.
+// Lockstep groups of G blocks, G = 8 when the groups fit one per CU, else 12.
+// A round costs the solver work of the busiest SIMD (the waves of a group are
+// spread over the CU's 4 SIMDs) plus the network's weight stream, whose time
+// grows with the number of CUs streaming from L2 at once.  G = 8 puts 2 waves
+// on every SIMD; G = 12 puts 3 on every SIMD, so G = 10 (3, 3, 2, 2) costs the
+// same solver time as 12 with more groups streaming weights, and is not used.
+// Returns G (> 0) or a negative error.
 extern "C" int cm_launch_bm_ws(const cm::BmLaunch *a, const cm::WsArgs *w, int n_cus, hipStream_t stream) {
   if (a->sc.jacobi_mode != 1 || w->net_every < 1) return -1;
   const int n = a->n_items;
-  int G = 12;
-  if ((n + 7) / 8 <= n_cus) G = 8;
-  else if ((n + 9) / 10 <= n_cus) G = 10;
+  const int G = (n + 7) / 8 <= n_cus ? 8 : 12;
   const dim3 grid((n + G - 1) / G);
-  switch (G) {
-    case 8: hipLaunchKernelGGL((cm::bm_ws_kernel<8>), grid, dim3(64 * 8), 0, stream, *a, *w); break;
-    case 10: hipLaunchKernelGGL((cm::bm_ws_kernel<10>), grid, dim3(64 * 10), 0, stream, *a, *w); break;
-    default: hipLaunchKernelGGL((cm::bm_ws_kernel<12>), grid, dim3(64 * 12), 0, stream, *a, *w); break;
-  }
+  if (G == 8)
+    hipLaunchKernelGGL((cm::bm_ws_kernel<8>), grid, dim3(64 * 8), 0, stream, *a, *w);
+  else
+    hipLaunchKernelGGL((cm::bm_ws_kernel<12>), grid, dim3(64 * 12), 0, stream, *a, *w);
   return hipGetLastError() == hipSuccess ? G : -2;
 }
