@@ -47,3 +47,38 @@ def test_host_model_calls_without_gpu():
     prof, refs, serial, calc, to_ref = stages.jnd_profiles([1000.0, 2000.0], [10.0, 20.0], noises=(1111.0, 40.0))
     assert len(prof) == 10 and list(refs) == [4, 9]
     assert all(to_ref[serial[i]] == (4 if i < 5 else 9) for i in calc)
+
+
+def test_bm_run_rejects_invalid_arguments_before_any_launch():
+    """cm_bm_run validates its arguments on the host: null buffers, negative
+    counts, pass and sweep limits, warm-start modes and the disabled
+    persistent launch are refused with a status and a message, and an empty
+    batch is a no-op.  Every case returns before a device call, so this runs
+    without a GPU."""
+    import ctypes as C
+    from cochlearmodel_amd import _lib, solver
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built")
+    L = _lib.lib()
+    fake = C.c_void_p(0x1000)   # never dereferenced: every case below returns first
+
+    def run(sc, n_items=1, inp=fake, persistent=0):
+        return L.cm_bm_run(C.byref(sc), inp, 16, fake, fake, fake, n_items, fake, 12, None, None, None, None, None,
+                           None, None, None, None, 1, persistent, 0)
+
+    def msg():
+        return L.cm_last_error().decode()
+
+    sc = solver.default_scalars()
+    assert run(sc, inp=None) == -1 and "null argument" in msg()
+    assert run(sc, n_items=-1) == -1
+    assert run(sc, persistent=1) == -3 and "disabled" in msg()
+    for field, value in (("j_fast", -1), ("j_slow", -1), ("max_passes", 0)):
+        bad = solver.default_scalars()
+        setattr(bad, field, value)
+        assert run(bad) == -3 and "Jacobi/pass" in msg(), field
+    for ws, check_every in ((3, 4), (-1, 4), (1, 0)):
+        bad = solver.default_scalars()
+        bad.jacobi_mode, bad.warm_start, bad.check_every, bad.tol_rel, bad.max_sweeps = 1, ws, check_every, 1e-6, 1000
+        assert run(bad) == -3 and "warm_start" in msg(), (ws, check_every)
+    assert run(sc, n_items=0) == 0

@@ -365,3 +365,46 @@ def test_hearing_aid_transfer_functions(oracle):
     rel = np.linalg.norm(got - ref) / np.linalg.norm(ref)
     _say(f"hearing aid vs float64 restatement: rel L2 {rel:.2e}")
     assert rel <= 1e-6
+
+
+@pytest.mark.parametrize("mode", ["reference", "plain", "extrapolated", "learned"])
+def test_zero_input_gives_exact_zero_in_every_mode(oracle, mode):
+    """Silence in, exact zeros out: BM velocity and pressure never leave 0 in
+    the reference method and in tolerance mode with each guess (the residual of
+    a zero right-hand side is zero, so no sweep runs).  Step, pass and sweep
+    counts equal the oracle's on the same silent blocks; the learned guess is
+    refused by its residual gate and matches the extrapolated run."""
+    import torch
+    from cochlearmodel_amd import solver, warmstart
+    _say(f"case: silent input, 3 blocks of 441 samples, {mode}")
+    n = 441
+    inp = np.zeros(3 * n + 2, np.float32)
+    items = np.zeros(3, solver.ITEM_DTYPE)
+    for k in range(3):
+        items[k] = (k * n, k * n, (k + 1) * n, k * n, 1e-10, 1e-15, 0, 0)
+    sc = solver.default_scalars(FS)
+    so = oracle.default_scalars(FS)
+    if mode != "reference":
+        ws = {"plain": 0, "extrapolated": 1, "learned": 2}[mode]
+        sc.jacobi_mode, sc.tol_rel, sc.check_every, sc.max_sweeps, sc.warm_start = 1, 1e-6, 4, 1000, ws
+        so.jacobi_mode, so.tol_rel, so.check_every, so.max_sweeps, so.warm_start = 1, 1e-6, 4, 1000, min(ws, 1)
+    if mode == "learned":
+        params, c_out, _ = warmstart.load_model()
+        warmstart.upload(params, c_out)
+    bb = solver.BatchedBM(np.full(256, 0.5, np.float32))
+    d_speeds = torch.full((3 * n, 256), 7.0, dtype=torch.float32, device="cuda")
+    out = bb.run(sc, torch.from_numpy(inp).cuda(), items, 3 * n, diagnostics=True, d_speeds=d_speeds)
+    v = out["speeds"].cpu().numpy()
+    st = out["stats"].cpu().numpy()
+    assert np.all(v == 0.0), "silent input must give exact zeros"
+    assert np.all(out["failed_nodes"].cpu().numpy() == 0)
+    it = np.zeros(3, oracle.ITEM_DTYPE)
+    for f in ("start", "mosi", "mesi", "write_start", "tol", "m1tol", "set"):
+        it[f] = items[f]
+    o = oracle.bm_run(so, oracle.section_params(), np.full(256, 0.5, np.float32), it, inp, 3 * n, "f32", threads=3)
+    _say(f"steps gpu {st[:, 0]} oracle {o['steps']}, sweeps gpu {st[:, 2]} oracle {o['sweeps']}")
+    np.testing.assert_array_equal(st[:, 0], o["steps"])
+    np.testing.assert_array_equal(st[:, 1], o["passes"])
+    np.testing.assert_array_equal(st[:, 2], o["sweeps"])
+    if mode != "reference":
+        assert np.all(st[:, 2] == 0)
