@@ -632,6 +632,7 @@ __global__ __launch_bounds__(64 * G) void bm_ws_kernel(BmLaunch a, WsArgs w) {
   __shared__ __align__(16) uint16_t H1s[16 * WS_HS];
   __shared__ __align__(16) uint16_t H2s[16 * WS_HS];      // also the output tile (16 x WS_OS floats)
   __shared__ float4 pold_lds[G * 65];
+  __shared__ float4 keep_lds[G * 5 * 64];                  // predictor right-hand side across the network
   __shared__ int active_lds[2][G];                         // double-buffered by round parity
 
   const int lane = threadIdx.x & 63;
@@ -640,6 +641,7 @@ __global__ __launch_bounds__(64 * G) void bm_ws_kernel(BmLaunch a, WsArgs w) {
   const bool valid = slot < a.n_items;
   const int idx = valid ? (a.order ? a.order[slot] : slot) : slot;   // item integrated by this wave
   float4 *pold = pold_lds + wave * 65;
+  float4 *keep = keep_lds + wave * 5 * 64;   // Y, od, G, nxis, Ptm of the predictor pass
 
   // zero the feature tile once (rows >= G stay zero)
   for (int e = threadIdx.x; e < 16 * WS_XS; e += 64 * G) Xs[e] = 0;
@@ -669,11 +671,17 @@ __global__ __launch_bounds__(64 * G) void bm_ws_kernel(BmLaunch a, WsArgs w) {
       float od[4], G4[4], Y[4];
       pass_begin(s, true);
       const float sf = pass_rhs(k, c, s, true, lane, od, G4, Y);
+      keep[lane] = make_float4(Y[0], Y[1], Y[2], Y[3]);
+      keep[64 + lane] = make_float4(od[0], od[1], od[2], od[3]);
+      keep[128 + lane] = make_float4(G4[0], G4[1], G4[2], G4[3]);
+      keep[192 + lane] = make_float4(s.nxis[0], s.nxis[1], s.nxis[2], s.nxis[3]);
+      keep[256 + lane] = make_float4(s.Ptm[0], s.Ptm[1], s.Ptm[2], s.Ptm[3]);
+      if (lane == 0) reinterpret_cast<float *>(pold + 64)[1] = sf;
       rr0 = extrapolate_guess(k, Y, s.dt, pold, lane, s.p, rre, true);   // s.p = p_lin
-      float r[4];
-      resid_vec(Y, k.mf, k.low23, s.p, r);
       rr_scale = __builtin_sqrtf(rre * inv_256) + 1e-30f;
       if (net_round || w.capture) {
+      float r[4];
+      resid_vec(Y, k.mf, k.low23, s.p, r);
       const float4 pk = pold[lane];   // plain guess (previous pressure)
       const float inv_d = 1.0f / rms256(s.nd[0], s.nd[1], s.nd[2], s.nd[3]);
       const float inv_v = 1.0f / rms256(s.nv[0], s.nv[1], s.nv[2], s.nv[3]);
@@ -735,11 +743,21 @@ __global__ __launch_bounds__(64 * G) void bm_ws_kernel(BmLaunch a, WsArgs w) {
     asm volatile("" ::: "memory");
     load_consts<true>(a, it, lane, k);
     if (active) {
-      // the predictor state and right-hand side again (same inputs, same
-      // values): cheaper than holding them in registers across the network
+      // the predictor state again (same inputs, same values) and its
+      // right-hand side from LDS: cheaper than holding either in registers
+      // across the network
       float od[4], G4[4], Y[4];
       pass_begin(s, true);
-      const float sf = pass_rhs(k, c, s, true, lane, od, G4, Y);
+      {
+        const float4 y = keep[lane], o = keep[64 + lane], g = keep[128 + lane], x = keep[192 + lane],
+                     t = keep[256 + lane];
+        Y[0] = y.x; Y[1] = y.y; Y[2] = y.z; Y[3] = y.w;
+        od[0] = o.x; od[1] = o.y; od[2] = o.z; od[3] = o.w;
+        G4[0] = g.x; G4[1] = g.y; G4[2] = g.z; G4[3] = g.w;
+        s.nxis[0] = x.x; s.nxis[1] = x.y; s.nxis[2] = x.z; s.nxis[3] = x.w;
+        s.Ptm[0] = t.x; s.Ptm[1] = t.y; s.Ptm[2] = t.z; s.Ptm[3] = t.w;
+      }
+      const float sf = reinterpret_cast<const float *>(pold + 64)[1];
       const float yy = tree_sum(Y[0] * Y[0], Y[1] * Y[1], Y[2] * Y[2], Y[3] * Y[3]);
       float rr_sel = rre;   // residual of the guess kept (p_lin unless replaced below)
       if (net_round) {
