@@ -557,24 +557,37 @@ typedef uint32_t ws_u32x2 __attribute__((ext_vector_type(2)));
 
 // NTL (1..3) column tiles t0, t0 + G, t0 + 2G of one layer, sharing the A
 // fragments of each k step.  Weights come as 16-byte pairs of k tiles (uint4 =
-// fragments of k tiles 2j and 2j+1); the unrolled loop keeps several pairs of
-// every tile in flight.
+// fragments of k tiles 2j and 2j+1); a ring of D pairs per tile stays in
+// flight: pair kp + D is requested as soon as pair kp has been consumed.  The
+// ring is one register array of WS_RING entries, slot (j, i) at j * NTL + i.
+constexpr int WS_RING = 9;
+template <int NTL>
+struct WsDepth {
+  static constexpr int D = NTL == 3 ? 3 : (NTL == 2 ? 4 : 6);
+};
+
+// First D pairs of every tile: global loads only, so a wave can issue them
+// before the barrier that publishes the layer's input tile.
+template <int G, int KT, int NT, int NTL>
+__device__ __forceinline__ void ws_fill(ws_u32x4 (&ring)[WS_RING], const CM_GLB ws_u32x4 *W, int lane, int t0) {
+  constexpr int KP = (KT + 1) / 2, D = WsDepth<NTL>::D;
+  const CM_GLB ws_u32x4 *wl = W + t0 * 64 + lane;
+  UNROLL for (int j = 0; j < D; ++j)
+    if (j < KP) {
+      UNROLL for (int i = 0; i < NTL; ++i) ring[j * NTL + i] = wl[(j * NT + i * G) * 64];
+    }
+}
+
+// The k loop over a filled ring and the bias / activation epilogue.
 template <int G, int KT, int NT, bool GELU, int NTL>
-__device__ __forceinline__ void ws_tiles(const CM_LDS uint16_t *A, int sa, const CM_GLB ws_u32x4 *W,
-                                         const CM_GLB float *bias, CM_LDS void *out, int so, int lane, int t0) {
-  constexpr int KP = (KT + 1) / 2;
+__device__ __forceinline__ void ws_run(ws_u32x4 (&ring)[WS_RING], const CM_LDS uint16_t *A, int sa,
+                                       const CM_GLB ws_u32x4 *W, const CM_GLB float *bias, CM_LDS void *out, int so,
+                                       int lane, int t0) {
+  constexpr int KP = (KT + 1) / 2, D = WsDepth<NTL>::D;
   ws_f32x4 acc[NTL];
   UNROLL for (int i = 0; i < NTL; ++i) acc[i] = ws_f32x4{0.f, 0.f, 0.f, 0.f};
   const CM_GLB ws_u32x4 *wl = W + t0 * 64 + lane;
   const CM_LDS uint16_t *arow = A + (lane & 15) * sa + 4 * (lane >> 4);
-  // ring of D k-tile pairs in flight per tile: pair kp + D is requested as
-  // soon as pair kp has been consumed
-  constexpr int D = NTL == 3 ? 3 : (NTL == 2 ? 4 : 6);
-  ws_u32x4 ring[D][NTL];
-  UNROLL for (int j = 0; j < D; ++j)
-    if (j < KP) {
-      UNROLL for (int i = 0; i < NTL; ++i) ring[j][i] = wl[(j * NT + i * G) * 64];
-    }
   __builtin_amdgcn_sched_barrier(0);
   UNROLL for (int kp = 0; kp < KP; ++kp) {
     const int slot = kp % D;
@@ -582,12 +595,12 @@ __device__ __forceinline__ void ws_tiles(const CM_LDS uint16_t *A, int sa, const
     const ws_bf16x4 a1 =
         __builtin_bit_cast(ws_bf16x4, *reinterpret_cast<const CM_LDS ws_u32x2 *>(arow + 32 * kp + 16));
     UNROLL for (int i = 0; i < NTL; ++i) {
-      const ws_u32x2 lo = ring[slot][i].xy, hi = ring[slot][i].zw;
+      const ws_u32x2 lo = ring[slot * NTL + i].xy, hi = ring[slot * NTL + i].zw;
       acc[i] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a0, __builtin_bit_cast(ws_bf16x4, lo), acc[i], 0, 0, 0);
       acc[i] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1, __builtin_bit_cast(ws_bf16x4, hi), acc[i], 0, 0, 0);
     }
     if (kp + D < KP) {
-      UNROLL for (int i = 0; i < NTL; ++i) ring[slot][i] = wl[((kp + D) * NT + i * G) * 64];
+      UNROLL for (int i = 0; i < NTL; ++i) ring[slot * NTL + i] = wl[((kp + D) * NT + i * G) * 64];
     }
     __builtin_amdgcn_sched_barrier(0);   // keep the requests D pairs ahead of their use
   }
@@ -607,18 +620,41 @@ __device__ __forceinline__ void ws_tiles(const CM_LDS uint16_t *A, int sa, const
 }
 
 // One dense layer out[16][N] = act(A[16][K] W + b), N = 16 NT: wave w computes
-// column tiles w, w + G, w + 2G, then w + 3G, ... three at a time.
+// column tiles w, w + G, w + 2G, then w + 3G, ... three at a time.  Every wave
+// has a first group of tiles (G < NT); its weight ring is filled before the
+// layer's barrier, which every wave then passes exactly once, so the first
+// requests are in flight while the slowest wave finishes the previous stage.
 template <int G, int KT, int NT, bool GELU>
 __device__ __noinline__ void ws_layer(const CM_LDS uint16_t *A, int sa, const CM_GLB ws_u32x4 *W,
                                       const CM_GLB float *bias, CM_LDS void *out, int so, int wave, int lane) {
-  for (int t0 = wave; t0 < NT; t0 += 3 * G) {
+  static_assert(G < NT, "every wave needs a first tile group");
+  ws_u32x4 ring[WS_RING];
+  const int n0 = wave + 2 * G < NT ? 3 : (wave + G < NT ? 2 : 1);
+  if (n0 == 3)
+    ws_fill<G, KT, NT, 3>(ring, W, lane, wave);
+  else if (n0 == 2)
+    ws_fill<G, KT, NT, 2>(ring, W, lane, wave);
+  else
+    ws_fill<G, KT, NT, 1>(ring, W, lane, wave);
+  __syncthreads();
+  if (n0 == 3)
+    ws_run<G, KT, NT, GELU, 3>(ring, A, sa, W, bias, out, so, lane, wave);
+  else if (n0 == 2)
+    ws_run<G, KT, NT, GELU, 2>(ring, A, sa, W, bias, out, so, lane, wave);
+  else
+    ws_run<G, KT, NT, GELU, 1>(ring, A, sa, W, bias, out, so, lane, wave);
+  for (int t0 = wave + 3 * G; t0 < NT; t0 += 3 * G) {
     const int n = t0 + 2 * G < NT ? 3 : (t0 + G < NT ? 2 : 1);
-    if (n == 3)
-      ws_tiles<G, KT, NT, GELU, 3>(A, sa, W, bias, out, so, lane, t0);
-    else if (n == 2)
-      ws_tiles<G, KT, NT, GELU, 2>(A, sa, W, bias, out, so, lane, t0);
-    else
-      ws_tiles<G, KT, NT, GELU, 1>(A, sa, W, bias, out, so, lane, t0);
+    if (n == 3) {
+      ws_fill<G, KT, NT, 3>(ring, W, lane, t0);
+      ws_run<G, KT, NT, GELU, 3>(ring, A, sa, W, bias, out, so, lane, t0);
+    } else if (n == 2) {
+      ws_fill<G, KT, NT, 2>(ring, W, lane, t0);
+      ws_run<G, KT, NT, GELU, 2>(ring, A, sa, W, bias, out, so, lane, t0);
+    } else {
+      ws_fill<G, KT, NT, 1>(ring, W, lane, t0);
+      ws_run<G, KT, NT, GELU, 1>(ring, A, sa, W, bias, out, so, lane, t0);
+    }
   }
 }
 
@@ -720,20 +756,24 @@ __global__ __launch_bounds__(64 * G) void bm_ws_kernel(BmLaunch a, WsArgs w) {
     // a fast wave may write the next round's flag before a slow one has read
     // this round's, so the flags alternate between two slots
     if (lane == 0) active_lds[round & 1][wave] = active ? 1 : 0;
-    __syncthreads();
+    // X -> H2 -> H1 -> output tile in H2's space (H2 is dead once layer 2 ends).
+    // Each layer's barrier sits inside ws_layer, after the first weight
+    // requests; the first one also publishes the activity flags.  When no
+    // block of the group is left the first layer runs once more on a stale
+    // tile, and its result is never read.
+    typedef const CM_LDS uint16_t *lds_in;
+    typedef CM_LDS void *lds_out;
+    typedef const CM_GLB ws_u32x4 *glb_w;
+    typedef const CM_GLB float *glb_b;
+    if (net_round)
+      ws_layer<G, 49, 32, true>((lds_in)Xs, WS_XS, (glb_w)w.w1, (glb_b)w.b1, (lds_out)H2s, WS_HS, wave, lane);
+    else
+      __syncthreads();
     int any = 0;
     for (int g = 0; g < G; ++g) any |= active_lds[round & 1][g];
     if (!any) break;
     if (net_round) {
-      // X -> H2 -> H1 -> output tile in H2's space (H2 is dead once layer 2 ends)
-      typedef const CM_LDS uint16_t *lds_in;
-      typedef CM_LDS void *lds_out;
-      typedef const CM_GLB ws_u32x4 *glb_w;
-      typedef const CM_GLB float *glb_b;
-      ws_layer<G, 49, 32, true>((lds_in)Xs, WS_XS, (glb_w)w.w1, (glb_b)w.b1, (lds_out)H2s, WS_HS, wave, lane);
-      __syncthreads();
       ws_layer<G, 32, 32, true>((lds_in)H2s, WS_HS, (glb_w)w.w2, (glb_b)w.b2, (lds_out)H1s, WS_HS, wave, lane);
-      __syncthreads();
       ws_layer<G, 32, 16, false>((lds_in)H1s, WS_HS, (glb_w)w.w3, (glb_b)w.b3, (lds_out)H2s, WS_OS, wave, lane);
       __syncthreads();
     }
