@@ -122,12 +122,13 @@ def test_exact_targets_match_dense_solve():
 
 
 @pytest.mark.parametrize("n,dur,per_item,group", [(4, 0.04, 64, 8), (2040, 0.02, 4, 8), (2200, 0.02, 4, 12),
-                                                   (2700, 0.02, 4, 12)])
+                                                   (2700, 0.02, 4, 12), (3300, 0.02, 4, 12)])
 def test_in_solver_network_matches_oracle(n, dur, per_item, group):
     """The MFMA inference inside the solver (fragment packing, tile mapping,
     GELU, bf16 operands) against the numpy oracle on the captured inputs, for
     both lockstep group sizes (8 blocks when the groups fit one per CU of 256,
-    else 12), with full and partly filled last groups."""
+    else 12), with full and partly filled last groups, and more groups than
+    CUs (3300 blocks: 275 groups)."""
     _cuda()
     from cochlearmodel_amd import warmstart
     from cochlearmodel_amd.mlp import SIZES
