@@ -566,49 +566,27 @@ struct WsDepth {
   static constexpr int D = NTL == 3 ? 3 : (NTL == 2 ? 4 : 6);
 };
 
-// Weight fragments are read with buffer loads: the resource (base and byte
-// size of the layer's fragment array) and the uniform pair/tile offset live in
-// scalar registers, the lane's 16-byte slot is the only vector offset, so no
-// VALU address arithmetic is spent per load, and a load past the array reads
-// zero instead of faulting.  Fragment (pair kp, tile t) of lane l sits at byte
-// ((kp * NT + t) * 64 + l) * 16.
-constexpr int WS_RSRC_WORD3 = 0x00020000;   // raw 32-bit buffer, gfx9 family
-template <int KT, int NT>
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t ws_rsrc(const CM_GLB ws_u32x4 *W) {
-  constexpr int KP = (KT + 1) / 2;
-  // the base is the same for every lane; readfirstlane tells the compiler so
-  // (a function argument is otherwise taken as per-lane), keeping the
-  // resource in scalar registers
-  const uint64_t a = reinterpret_cast<uint64_t>(W);
-  const uint64_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)a);
-  const uint64_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(a >> 32));
-  return __builtin_amdgcn_make_buffer_rsrc(reinterpret_cast<void *>((hi << 32) | lo), 0, KP * NT * 1024,
-                                           WS_RSRC_WORD3);
-}
-template <int NT, int G>
-__device__ __forceinline__ ws_u32x4 ws_load(__amdgpu_buffer_rsrc_t r, int voff, int kp, int i) {
-  return __builtin_amdgcn_raw_buffer_load_b128(r, voff, (kp * NT + i * G) * 1024, 0);
-}
-
-// First D pairs of every tile: loads only, so a wave can issue them before the
-// barrier that publishes the layer's input tile.  voff = (t0 * 64 + lane) * 16.
+// First D pairs of every tile: global loads only, so a wave can issue them
+// before the barrier that publishes the layer's input tile.
 template <int G, int KT, int NT, int NTL>
-__device__ __forceinline__ void ws_fill(ws_u32x4 (&ring)[WS_RING], __amdgpu_buffer_rsrc_t r, int voff) {
+__device__ __forceinline__ void ws_fill(ws_u32x4 (&ring)[WS_RING], const CM_GLB ws_u32x4 *W, int lane, int t0) {
   constexpr int KP = (KT + 1) / 2, D = WsDepth<NTL>::D;
+  const CM_GLB ws_u32x4 *wl = W + t0 * 64 + lane;
   UNROLL for (int j = 0; j < D; ++j)
     if (j < KP) {
-      UNROLL for (int i = 0; i < NTL; ++i) ring[j * NTL + i] = ws_load<NT, G>(r, voff, j, i);
+      UNROLL for (int i = 0; i < NTL; ++i) ring[j * NTL + i] = wl[(j * NT + i * G) * 64];
     }
 }
 
 // The k loop over a filled ring and the bias / activation epilogue.
 template <int G, int KT, int NT, bool GELU, int NTL>
 __device__ __forceinline__ void ws_run(ws_u32x4 (&ring)[WS_RING], const CM_LDS uint16_t *A, int sa,
-                                       __amdgpu_buffer_rsrc_t r, int voff, const CM_GLB float *bias,
-                                       CM_LDS void *out, int so, int lane, int t0) {
+                                       const CM_GLB ws_u32x4 *W, const CM_GLB float *bias, CM_LDS void *out, int so,
+                                       int lane, int t0) {
   constexpr int KP = (KT + 1) / 2, D = WsDepth<NTL>::D;
   ws_f32x4 acc[NTL];
   UNROLL for (int i = 0; i < NTL; ++i) acc[i] = ws_f32x4{0.f, 0.f, 0.f, 0.f};
+  const CM_GLB ws_u32x4 *wl = W + t0 * 64 + lane;
   const CM_LDS uint16_t *arow = A + (lane & 15) * sa + 4 * (lane >> 4);
   __builtin_amdgcn_sched_barrier(0);
   UNROLL for (int kp = 0; kp < KP; ++kp) {
@@ -622,7 +600,7 @@ __device__ __forceinline__ void ws_run(ws_u32x4 (&ring)[WS_RING], const CM_LDS u
       acc[i] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1, __builtin_bit_cast(ws_bf16x4, hi), acc[i], 0, 0, 0);
     }
     if (kp + D < KP) {
-      UNROLL for (int i = 0; i < NTL; ++i) ring[slot * NTL + i] = ws_load<NT, G>(r, voff, kp + D, i);
+      UNROLL for (int i = 0; i < NTL; ++i) ring[slot * NTL + i] = wl[((kp + D) * NT + i * G) * 64];
     }
     __builtin_amdgcn_sched_barrier(0);   // keep the requests D pairs ahead of their use
   }
@@ -650,35 +628,32 @@ template <int G, int KT, int NT, bool GELU>
 __device__ __noinline__ void ws_layer(const CM_LDS uint16_t *A, int sa, const CM_GLB ws_u32x4 *W,
                                       const CM_GLB float *bias, CM_LDS void *out, int so, int wave, int lane) {
   static_assert(G < NT, "every wave needs a first tile group");
-  const __amdgpu_buffer_rsrc_t r = ws_rsrc<KT, NT>(W);
   ws_u32x4 ring[WS_RING];
   const int n0 = wave + 2 * G < NT ? 3 : (wave + G < NT ? 2 : 1);
-  const int voff0 = (wave * 64 + lane) * 16;
   if (n0 == 3)
-    ws_fill<G, KT, NT, 3>(ring, r, voff0);
+    ws_fill<G, KT, NT, 3>(ring, W, lane, wave);
   else if (n0 == 2)
-    ws_fill<G, KT, NT, 2>(ring, r, voff0);
+    ws_fill<G, KT, NT, 2>(ring, W, lane, wave);
   else
-    ws_fill<G, KT, NT, 1>(ring, r, voff0);
+    ws_fill<G, KT, NT, 1>(ring, W, lane, wave);
   __syncthreads();
   if (n0 == 3)
-    ws_run<G, KT, NT, GELU, 3>(ring, A, sa, r, voff0, bias, out, so, lane, wave);
+    ws_run<G, KT, NT, GELU, 3>(ring, A, sa, W, bias, out, so, lane, wave);
   else if (n0 == 2)
-    ws_run<G, KT, NT, GELU, 2>(ring, A, sa, r, voff0, bias, out, so, lane, wave);
+    ws_run<G, KT, NT, GELU, 2>(ring, A, sa, W, bias, out, so, lane, wave);
   else
-    ws_run<G, KT, NT, GELU, 1>(ring, A, sa, r, voff0, bias, out, so, lane, wave);
+    ws_run<G, KT, NT, GELU, 1>(ring, A, sa, W, bias, out, so, lane, wave);
   for (int t0 = wave + 3 * G; t0 < NT; t0 += 3 * G) {
     const int n = t0 + 2 * G < NT ? 3 : (t0 + G < NT ? 2 : 1);
-    const int voff = (t0 * 64 + lane) * 16;
     if (n == 3) {
-      ws_fill<G, KT, NT, 3>(ring, r, voff);
-      ws_run<G, KT, NT, GELU, 3>(ring, A, sa, r, voff, bias, out, so, lane, t0);
+      ws_fill<G, KT, NT, 3>(ring, W, lane, t0);
+      ws_run<G, KT, NT, GELU, 3>(ring, A, sa, W, bias, out, so, lane, t0);
     } else if (n == 2) {
-      ws_fill<G, KT, NT, 2>(ring, r, voff);
-      ws_run<G, KT, NT, GELU, 2>(ring, A, sa, r, voff, bias, out, so, lane, t0);
+      ws_fill<G, KT, NT, 2>(ring, W, lane, t0);
+      ws_run<G, KT, NT, GELU, 2>(ring, A, sa, W, bias, out, so, lane, t0);
     } else {
-      ws_fill<G, KT, NT, 1>(ring, r, voff);
-      ws_run<G, KT, NT, GELU, 1>(ring, A, sa, r, voff, bias, out, so, lane, t0);
+      ws_fill<G, KT, NT, 1>(ring, W, lane, t0);
+      ws_run<G, KT, NT, GELU, 1>(ring, A, sa, W, bias, out, so, lane, t0);
     }
   }
 }
