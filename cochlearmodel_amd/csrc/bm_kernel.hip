@@ -555,6 +555,21 @@ typedef __attribute__((__vector_size__(4 * sizeof(float)))) float ws_f32x4;
 typedef uint32_t ws_u32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t ws_u32x2 __attribute__((ext_vector_type(2)));
 
+// In-solver inference only: GELU (tanh form) as x * sigmoid(2u), u =
+// sqrt(2/pi) (x + 0.044715 x^3), with the hardware exp2 and reciprocal, and the
+// hardware round-to-nearest-even BF16 conversion.  Both agree with the exact
+// forms to a few FP32 ulps, far inside the BF16 rounding of the activations
+// (training keeps cm_gelu).
+__device__ __forceinline__ float ws_gelu(float x) {
+  const float t = x * x;
+  const float z = (x * __builtin_fmaf(0.044715f, t, 1.0f)) * -2.302208198f;   // -2 sqrt(2/pi) log2(e)
+  return x * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(z));
+}
+__device__ __forceinline__ uint16_t ws_bf16(float x) { return __builtin_bit_cast(uint16_t, (__bf16)x); }
+__device__ __forceinline__ uint32_t ws_bf16x2(float lo, float hi) {
+  return (uint32_t)ws_bf16(lo) | ((uint32_t)ws_bf16(hi) << 16);
+}
+
 // NTL (1..3) column tiles t0, t0 + G, t0 + 2G of one layer, sharing the A
 // fragments of each k step.  Weights come as 16-byte pairs of k tiles (uint4 =
 // fragments of k tiles 2j and 2j+1); a ring of D pairs per tile stays in
@@ -612,7 +627,7 @@ __device__ __forceinline__ void ws_run(ws_u32x4 (&ring)[WS_RING], const CM_LDS u
     UNROLL for (int q = 0; q < 4; ++q) {
       const float v = acc[i][q] + bv;
       if (GELU)
-        reinterpret_cast<CM_LDS uint16_t *>(out)[(r0 + q) * so + col] = cm_f32_to_bf16(cm_gelu(v));
+        reinterpret_cast<CM_LDS uint16_t *>(out)[(r0 + q) * so + col] = ws_bf16(ws_gelu(v));
       else
         reinterpret_cast<CM_LDS float *>(out)[(r0 + q) * so + col] = v;
     }
@@ -731,12 +746,10 @@ __global__ __launch_bounds__(64 * G) void bm_ws_kernel(BmLaunch a, WsArgs w) {
       }
       f[12] = sf * inv_p;
       uint16_t *xr = Xs + wave * WS_XS;
-      UNROLL for (int j = 0; j < 4; ++j) {
-        xr[4 * lane + j] = cm_f32_to_bf16(f[j]);
-        xr[256 + 4 * lane + j] = cm_f32_to_bf16(f[4 + j]);
-        xr[512 + 4 * lane + j] = cm_f32_to_bf16(f[8 + j]);
-      }
-      if (lane == 0) xr[768] = cm_f32_to_bf16(f[12]);
+      UNROLL for (int b = 0; b < 3; ++b)
+        *reinterpret_cast<uint2 *>(xr + 256 * b + 4 * lane) =
+            make_uint2(ws_bf16x2(f[4 * b], f[4 * b + 1]), ws_bf16x2(f[4 * b + 2], f[4 * b + 3]));
+      if (lane == 0) xr[768] = ws_bf16(f[12]);
       if (w.capture && (s.n_steps % w.stride) == 0 && s.n_steps / w.stride < w.per_item) {   // slot of this step
         float *rec = w.records + ((int64_t)idx * w.per_item + s.n_steps / w.stride) * CM_WS_RECORD;
         *reinterpret_cast<float4 *>(rec + 4 * lane) = make_float4(f[0], f[1], f[2], f[3]);
