@@ -673,8 +673,10 @@ __device__ __noinline__ void ws_layer(const CM_LDS uint16_t *A, int sa, const CM
   }
 }
 
+// RMS over the 256 sections, for the network's input scales (the hardware
+// square root, 1 ulp, is enough there)
 __device__ __forceinline__ float rms256(float x0, float x1, float x2, float x3) {
-  return __builtin_sqrtf(tree_sum(x0 * x0, x1 * x1, x2 * x2, x3 * x3) * (1.0f / 256.0f)) + 1e-30f;
+  return __builtin_amdgcn_sqrtf(tree_sum(x0 * x0, x1 * x1, x2 * x2, x3 * x3) * (1.0f / 256.0f)) + 1e-30f;
 }
 
 template <int G>
@@ -729,15 +731,17 @@ __global__ __launch_bounds__(64 * G) void bm_ws_kernel(BmLaunch a, WsArgs w) {
       keep[256 + lane] = make_float4(s.Ptm[0], s.Ptm[1], s.Ptm[2], s.Ptm[3]);
       if (lane == 0) reinterpret_cast<float *>(pold + 64)[1] = sf;
       rr0 = extrapolate_guess(k, Y, s.dt, pold, lane, s.p, rre, true);   // s.p = p_lin
-      rr_scale = __builtin_sqrtf(rre * inv_256) + 1e-30f;
+      rr_scale = __builtin_amdgcn_sqrtf(rre * inv_256) + 1e-30f;
       if (net_round || w.capture) {
       float r[4];
       resid_vec(Y, k.mf, k.low23, s.p, r);
       const float4 pk = pold[lane];   // plain guess (previous pressure)
-      const float inv_d = 1.0f / rms256(s.nd[0], s.nd[1], s.nd[2], s.nd[3]);
-      const float inv_v = 1.0f / rms256(s.nv[0], s.nv[1], s.nv[2], s.nv[3]);
-      const float inv_r = 1.0f / rr_scale;
-      const float inv_p = 1.0f / rms256(pk.x, pk.y, pk.z, pk.w);
+      // the scales only normalise BF16 network inputs: the hardware reciprocal
+      // (1 ulp) stands in for an IEEE division
+      const float inv_d = __builtin_amdgcn_rcpf(rms256(s.nd[0], s.nd[1], s.nd[2], s.nd[3]));
+      const float inv_v = __builtin_amdgcn_rcpf(rms256(s.nv[0], s.nv[1], s.nv[2], s.nv[3]));
+      const float inv_r = __builtin_amdgcn_rcpf(rr_scale);
+      const float inv_p = __builtin_amdgcn_rcpf(rms256(pk.x, pk.y, pk.z, pk.w));
       float f[13];
       UNROLL for (int j = 0; j < 4; ++j) {
         f[j] = s.nd[j] * inv_d;
