@@ -195,3 +195,28 @@ def test_trained_network_gains_over_plain_and_extrapolated(oracle):
     assert err["learned"] <= max(1.5 * err["plain"], 1e-3)
     assert err["learned_every2"] <= max(1.5 * err["plain"], 1e-3)
     assert abs(int(res["learned"][2]) - int(res["plain"][2])) <= 0.01 * res["plain"][2]
+
+
+def test_lockstep_grouping_does_not_change_results():
+    """Which blocks share a lockstep group (and so a network batch) is a launch
+    choice: the network's rows are independent and every block integrates on
+    its own, so an order table that regroups the blocks leaves speeds and
+    counters bit-identical."""
+    _cuda()
+    from cochlearmodel_amd import _lib, warmstart
+    _say("case: learned warm start, 40 blocks in natural and reversed-interleaved order")
+    params, c_out, _ = warmstart.load_model()
+    warmstart.upload(params, c_out)
+    tb = _batch(n=8, dur=0.1, seed=13)
+    sc = warmstart.tol_scalars(1e-6, 2)
+    v1, s1 = _run(tb, sc)
+    order = np.concatenate([np.arange(tb.TB)[::-2], np.arange(tb.TB)[-2::-2]]).astype(np.int32)
+    assert sorted(order.tolist()) == list(range(tb.TB))
+    L = _lib.lib()
+    _lib.check(L.cm_set_bm_order(_lib.ptr(order), len(order)))
+    try:
+        v2, s2 = _run(tb, sc)
+    finally:
+        _lib.check(L.cm_set_bm_order(None, 0))
+    np.testing.assert_array_equal(v1, v2)
+    np.testing.assert_array_equal(s1, s2)
