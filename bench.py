@@ -257,8 +257,17 @@ def c3_main(args):
         ub.run_bm(sc, timed=False)
         g = ub.d_speeds.index_select(0, d_rows).cpu().numpy()
         ref = o["speeds"][np.concatenate(crows)]
+        # one thread on the first sixteenth of the sample: the per-core rate
+        n1 = max(1, len(sel) // 16)
+        t1 = time.perf_counter()
+        o1 = O.bm_run(O.default_scalars(FS), O.section_params(), ohc, it[:n1], cin, off, "f32", threads=1)
+        dt1 = time.perf_counter() - t1
+        st1 = int(o1["steps"].sum())
         out["cpu_baseline"] = {"value": SECTIONS * st / dt, "unit": "section-steps/s", "cores": threads,
                                "kind": "port", "solver": "reference_20_2",
+                               "single_thread": {"value": SECTIONS * st1 / dt1,
+                                                 "sample": f"{n1} of the sampled blocks, {st1} committed steps, "
+                                                           f"{dt1:.2f} s wall, one thread"},
                                "sample": f"{len(sel)} of {ub.TB} time blocks (every {max(1, ub.TB // args.cpu_blocks)}"
                                          f"th), {st} committed steps, {dt:.2f} s wall, FP32 oracle, OpenMP over "
                                          "blocks",
@@ -388,6 +397,10 @@ def c4_main(args):
                                         np.array([0, 0], np.int32))
             db_cpu[int(j)] = (ai[0], rate[0])
         dt = time.perf_counter() - t1
+        # one thread on one calculated interval (BM + IHC/lambda): the per-core rate
+        t2 = time.perf_counter()
+        st1 = c4_oracle_interval(sw, int(pick[0]), 1)[2]
+        dt1 = time.perf_counter() - t2
         # GPU rows of the same intervals (the last step's outputs)
         dv, dl, dmax = 0.0, 0.0, 0.0
         nv = nl = 0.0
@@ -409,7 +422,11 @@ def c4_main(args):
                                          f"{dt:.2f} s wall, FP32 oracle, OpenMP over blocks and sections",
                                "parity_rel_l2_bm": (dv / max(nv, 1e-300)) ** 0.5,
                                "parity_rel_l2_lambda": (dl / max(nl, 1e-300)) ** 0.5,
-                               "parity_max_jnd_db": dmax, **host_cpu()}
+                               "parity_max_jnd_db": dmax,
+                               "single_thread": {"value": SECTIONS * st1 / dt1,
+                                                 "sample": f"interval {int(pick[0])} (BM + IHC/lambda), {st1} committed "
+                                                           f"steps, {dt1:.2f} s wall, one thread"},
+                               **host_cpu()}
     print(json.dumps(out))
 
 
