@@ -163,6 +163,57 @@ def cpu_baseline(work, n_blocks: int, threads: int, sc_gpu):
                       f"{dt:.2f} s wall, FP32 oracle of the same solver settings with OpenMP over blocks"}, o, sel
 
 
+def c1_main(args):
+    """One 1 kHz, 60 dB tone through the reference's VOICE call sequence
+    (BMOHCKernel_Init, BMOHCNewKernel, copies): 5 time blocks of 20 ms.  The
+    BM time is the kernel's own event timing; end to end adds the host copies."""
+    import torch
+    from cochlearmodel_amd import solver, workloads
+    from cochlearmodel_amd._lib import lib
+    lay = solver.voice_layout(FS, 5, 0.02, 0.005)
+    sig = workloads.c1_signal()
+    for _ in range(args.warmup):
+        solver.run_bm_voice(sig, lay, timed=True)
+    bm, e2e = [], []
+    g = None
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        g = solver.run_bm_voice(sig, lay, timed=True)
+        e2e.append((time.perf_counter() - t0) * 1e3)
+        bm.append(lib().cm_last_kernel_ms())
+    steps = int(g["stats"]["steps"].sum())
+    bm_avg = float(np.mean(bm))
+    out = {"metric": "section-steps/s (one VOICE tone)", "value": SECTIONS * steps / (bm_avg * 1e-3),
+           "unit": "section-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": float(np.mean(e2e)), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "1 kHz tone, 60 dB SPL, 100 ms with 10 ms ramps and 220 leading zeros, 44.1 kHz, "
+                                  "5 time blocks of 20 ms, reference method", "time_blocks": lay.time_blocks},
+           "bm_ms": bm_avg, "committed_steps": steps,
+           "note": "5 waves on a 256-CU chip: this case measures latency, not throughput"}
+    if not args.no_cpu:
+        from oracle import oracle as O
+        items = O.make_items(*O.block_decomposition(lay.time_blocks, lay.bufP1, lay.overlap_nodes, 0), 1e-10, 1e-15)
+        inp = g["input"]
+        res = {}
+        for threads in (1, lay.time_blocks):
+            t1 = time.perf_counter()
+            o = O.bm_run(O.default_scalars(FS), O.section_params(), np.full(SECTIONS, 0.5, np.float32), items, inp,
+                         lay.bufP1, "f32", threads=threads)
+            res[threads] = (time.perf_counter() - t1, int(o["steps"].sum()), o)
+        dt, st, o = res[lay.time_blocks]
+        v, vo = g["speeds"][:lay.bufP1], o["speeds"][:lay.bufP1]
+        out["cpu_baseline"] = {"value": SECTIONS * st / dt, "unit": "section-steps/s", "cores": lay.time_blocks,
+                               "kind": "port", "solver": "reference_20_2",
+                               "sample": f"the whole tone ({st} committed steps, {dt:.2f} s wall, one thread per "
+                                         "time block)",
+                               "single_thread": {"value": SECTIONS * res[1][1] / res[1][0],
+                                                 "sample": f"the whole tone, {res[1][0]:.2f} s wall, one thread"},
+                               "parity_rel_l2": float(np.linalg.norm(v - vo) / max(np.linalg.norm(vo), 1e-30)),
+                               **host_cpu()}
+    print(json.dumps(out))
+
+
 def c3_main(args):
     """Uneven batch: 1024 seeded signals (20-500 ms tones, 3-harmonic complexes
     and noise, 0-100 dB, per-signal OHC spline and IHC), reference method,
@@ -442,8 +493,9 @@ def main():
     ap.add_argument("--accuracy-blocks", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
-    ap.add_argument("--workload", default="c2", choices=("c2", "c3", "c4"),
-                    help="c2: 256 uniform tones (headline); c3: 1024 uneven signals; c4: JND sweep")
+    ap.add_argument("--workload", default="c2", choices=("c1", "c2", "c3", "c4"),
+                    help="c1: one VOICE tone; c2: 256 uniform tones (headline); c3: 1024 uneven signals; "
+                         "c4: JND sweep")
     ap.add_argument("--c3-signals", type=int, default=1024)
     ap.add_argument("--c4-intervals", type=int, default=8, help="calculated intervals checked on the CPU")
     ap.add_argument("--no-graph", action="store_true", help="c4: launch the post-BM stages eagerly")
@@ -464,6 +516,8 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    if args.workload == "c1":
+        return c1_main(args)
     if args.workload == "c3":
         return c3_main(args)
     if args.workload == "c4":
