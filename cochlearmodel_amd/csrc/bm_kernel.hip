@@ -545,6 +545,7 @@ constexpr int WS_HS = 520;    // row stride (bf16) of hidden tiles
 constexpr int WS_OS = 260;    // row stride (float) of the output tile
 
 typedef __attribute__((__vector_size__(4 * sizeof(short)))) short ws_bf16x4;
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 ws_bf16x8;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float ws_f32x4;
 
 // One dense layer: out[16][N] = act(A[16][K] W + b) with NT = N/16 column tiles
@@ -606,14 +607,15 @@ __device__ __forceinline__ void ws_run(ws_u32x4 (&ring)[WS_RING], const CM_LDS u
   __builtin_amdgcn_sched_barrier(0);
   UNROLL for (int kp = 0; kp < KP; ++kp) {
     const int slot = kp % D;
-    const ws_bf16x4 a0 = __builtin_bit_cast(ws_bf16x4, *reinterpret_cast<const CM_LDS ws_u32x2 *>(arow + 32 * kp));
-    const ws_bf16x4 a1 =
-        __builtin_bit_cast(ws_bf16x4, *reinterpret_cast<const CM_LDS ws_u32x2 *>(arow + 32 * kp + 16));
-    UNROLL for (int i = 0; i < NTL; ++i) {
-      const ws_u32x2 lo = ring[slot * NTL + i].xy, hi = ring[slot * NTL + i].zw;
-      acc[i] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a0, __builtin_bit_cast(ws_bf16x4, lo), acc[i], 0, 0, 0);
-      acc[i] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1, __builtin_bit_cast(ws_bf16x4, hi), acc[i], 0, 0, 0);
-    }
+    // one K = 32 MFMA per pair: A holds this lane's 4 inputs of k tile 2kp then
+    // its 4 of tile 2kp + 1, B the matching halves of the fragment pair, so the
+    // products summed are those of two K = 16 steps
+    const ws_u32x2 a0 = *reinterpret_cast<const CM_LDS ws_u32x2 *>(arow + 32 * kp);
+    const ws_u32x2 a1 = *reinterpret_cast<const CM_LDS ws_u32x2 *>(arow + 32 * kp + 16);
+    const ws_bf16x8 a8 = __builtin_bit_cast(ws_bf16x8, ws_u32x4{a0.x, a0.y, a1.x, a1.y});
+    UNROLL for (int i = 0; i < NTL; ++i)
+      acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a8, __builtin_bit_cast(ws_bf16x8, ring[slot * NTL + i]),
+                                                       acc[i], 0, 0, 0);
     if (kp + D < KP) {
       UNROLL for (int i = 0; i < NTL; ++i) ring[slot * NTL + i] = wl[((kp + D) * NT + i * G) * 64];
     }
