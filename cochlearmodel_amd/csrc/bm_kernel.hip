@@ -572,8 +572,9 @@ __device__ __forceinline__ uint32_t ws_bf16x2(float lo, float hi) {
 }
 
 // NTL (1..3) column tiles t0, t0 + G, t0 + 2G of one layer, sharing the A
-// fragments of each k step.  Weights come as 16-byte pairs of k tiles (uint4 =
-// fragments of k tiles 2j and 2j+1); a ring of D pairs per tile stays in
+// fragments of each k step.  Weights come as 16-byte fragments of k-tile pairs
+// (uint4 = this lane's 8 rows 32 j + 8 (lane / 16) + 0..7 of one column, the B
+// operand of a K = 32 MFMA); a ring of D pairs per tile stays in
 // flight: pair kp + D is requested as soon as pair kp has been consumed.  The
 // ring is one register array of WS_RING entries, slot (j, i) at j * NTL + i.
 constexpr int WS_RING = 9;
@@ -603,16 +604,14 @@ __device__ __forceinline__ void ws_run(ws_u32x4 (&ring)[WS_RING], const CM_LDS u
   ws_f32x4 acc[NTL];
   UNROLL for (int i = 0; i < NTL; ++i) acc[i] = ws_f32x4{0.f, 0.f, 0.f, 0.f};
   const CM_GLB ws_u32x4 *wl = W + t0 * 64 + lane;
-  const CM_LDS uint16_t *arow = A + (lane & 15) * sa + 4 * (lane >> 4);
+  const CM_LDS uint16_t *arow = A + (lane & 15) * sa + 8 * (lane >> 4);
   __builtin_amdgcn_sched_barrier(0);
   UNROLL for (int kp = 0; kp < KP; ++kp) {
     const int slot = kp % D;
-    // one K = 32 MFMA per pair: A holds this lane's 4 inputs of k tile 2kp then
-    // its 4 of tile 2kp + 1, B the matching halves of the fragment pair, so the
-    // products summed are those of two K = 16 steps
-    const ws_u32x2 a0 = *reinterpret_cast<const CM_LDS ws_u32x2 *>(arow + 32 * kp);
-    const ws_u32x2 a1 = *reinterpret_cast<const CM_LDS ws_u32x2 *>(arow + 32 * kp + 16);
-    const ws_bf16x8 a8 = __builtin_bit_cast(ws_bf16x8, ws_u32x4{a0.x, a0.y, a1.x, a1.y});
+    // one K = 32 MFMA per pair of k tiles: this lane's 8 consecutive inputs
+    // 32 kp + 8 (lane / 16) + 0..7 (one 16-byte LDS read) against the same 8
+    // rows of the weight fragment
+    const ws_bf16x8 a8 = __builtin_bit_cast(ws_bf16x8, *reinterpret_cast<const CM_LDS ws_u32x4 *>(arow + 32 * kp));
     UNROLL for (int i = 0; i < NTL; ++i)
       acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a8, __builtin_bit_cast(ws_bf16x8, ring[slot * NTL + i]),
                                                        acc[i], 0, 0, 0);

@@ -39,20 +39,19 @@ uint16_t host_bf16(float x) {
   return (uint16_t)(u >> 16);
 }
 
-// W row-major [K][N] -> [KT/2][NT][64 lanes] x 8 bf16: lane l of pair j, tile t
-// holds W[16 kt + 4 (l / 16) + q][16 t + l % 16] for kt = 2j (q = 0..3) then
-// kt = 2j + 1 (q = 0..3); zero past K.
+// W row-major [K][N] -> [KT/2][NT][64 lanes] x 8 bf16, the B operand of one
+// K = 32 MFMA per k-tile pair: lane l of pair j, tile t holds
+// W[32 j + 8 (l / 16) + e][16 t + l % 16] for e = 0..7; zero past K.
 std::vector<uint16_t> pack_layer(const float *W, int K, int N) {
   const int KT = (K + 15) / 16, KP = (KT + 1) / 2, NT = N / 16;
   std::vector<uint16_t> out((size_t)KP * NT * 64 * 8, 0);
   for (int kp = 0; kp < KP; ++kp)
     for (int nt = 0; nt < NT; ++nt)
       for (int l = 0; l < 64; ++l)
-        for (int h = 0; h < 2; ++h)
-          for (int q = 0; q < 4; ++q) {
-            const int k = 16 * (2 * kp + h) + 4 * (l / 16) + q, n = 16 * nt + l % 16;
-            out[(((size_t)kp * NT + nt) * 64 + l) * 8 + 4 * h + q] = k < K ? host_bf16(W[(size_t)k * N + n]) : 0;
-          }
+        for (int e = 0; e < 8; ++e) {
+          const int k = 32 * kp + 8 * (l / 16) + e, n = 16 * nt + l % 16;
+          out[(((size_t)kp * NT + nt) * 64 + l) * 8 + e] = k < K ? host_bf16(W[(size_t)k * N + n]) : 0;
+        }
   return out;
 }
 
