@@ -71,7 +71,7 @@ MLP_FLOP_PER_STEP = 2.0 * (769 * 512 + 512 * 512 + 512 * 256)   # forward, one b
 # WRITE_SIZE (KiB per dispatch), scaled by what the counters read for a 1 GiB
 # device copy (FETCH x 2.0, WRITE x 1.0); profiles/pmc_traffic_c2.md
 MEASURED_HBM_BYTES = {
-    "learned_tol": (32658.4375 * 2.0 + 15632105.0) * 1024,
+    "learned_tol": (50009.375 * 2.0 + 2265904.0) * 1024,
     "reference_20_2": (6083.44 * 2.0 + 2258260.0) * 1024,
     "ihc_lambda": (1150705.0 * 2.0 + 4515840.0 + 2289528.0 * 2.0 + 6773760.0) * 1024,
 }
@@ -80,12 +80,12 @@ PEAK_BF16_TFLOPS = 2500.0    # MI355X dense BF16 MFMA
 # profiles/pmc_valu_c2_reference.md, profiles/pmc_mix_c2_learned.md) and the issue
 # capacity they are measured against: a wave64 FP32 VALU instruction holds a SIMD
 # for 4 cycles, 1024 SIMDs at 2.4 GHz
-MEASURED_VALU_INSTS = {"reference_20_2": 1.0995e11, "learned_tol": 1.8354e11,
+MEASURED_VALU_INSTS = {"reference_20_2": 1.0995e11, "learned_tol": 1.6927e11,
                        "ihc_lambda": 1.40262e9 + 2.68728e9}   # ac_ds_kernel + dc_lambda_kernel
 VALU_ISSUE_PER_S = 1024 * 2.4e9 / 4
 # L1<-L2 reads of the learned kernel: TCP_TCC_READ_REQ_sum x 128 B (a 1 GiB copy
 # reads 8.389e6 requests); profiles/pmc_mix_c2_learned.md
-MEASURED_L2_READ_BYTES = {"learned_tol": 6.7334e10 * 128}
+MEASURED_L2_READ_BYTES = {"learned_tol": 6.6959e10 * 128}
 
 
 def load_warm_start_model():
