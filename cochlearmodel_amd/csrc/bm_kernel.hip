@@ -544,7 +544,6 @@ constexpr int WS_XS = 808;    // LDS row stride (bf16) of the feature tile (K = 
 constexpr int WS_HS = 520;    // row stride (bf16) of hidden tiles
 constexpr int WS_OS = 260;    // row stride (float) of the output tile
 
-typedef __attribute__((__vector_size__(4 * sizeof(short)))) short ws_bf16x4;
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 ws_bf16x8;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float ws_f32x4;
 
@@ -554,7 +553,6 @@ typedef __attribute__((__vector_size__(4 * sizeof(float)))) float ws_f32x4;
 #define CM_LDS __attribute__((address_space(3)))
 #define CM_GLB __attribute__((address_space(1)))
 typedef uint32_t ws_u32x4 __attribute__((ext_vector_type(4)));
-typedef uint32_t ws_u32x2 __attribute__((ext_vector_type(2)));
 
 // In-solver inference only: GELU (tanh form) as x * sigmoid(2u), u =
 // sqrt(2/pi) (x + 0.044715 x^3), with the hardware exp2 and reciprocal, and the
