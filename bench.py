@@ -83,6 +83,8 @@ PEAK_BF16_TFLOPS = 2500.0    # MI355X dense BF16 MFMA
 MEASURED_VALU_INSTS = {"reference_20_2": 1.0995e11, "learned_tol": 1.6927e11,
                        "ihc_lambda": 1.40262e9 + 2.68728e9}   # ac_ds_kernel + dc_lambda_kernel
 VALU_ISSUE_PER_S = 1024 * 2.4e9 / 4
+VALU_ISSUE_MODEL = ("1024 SIMDs x 2.4 GHz / 4 cycles per wave64 instruction; a lower bound: the BM kernel's mix "
+                    "issues 7.38e11/s at full occupancy (profiles/pmc_valu_c4_reference.md)")
 # L1<-L2 reads of the learned kernel: TCP_TCC_READ_REQ_sum x 128 B (a 1 GiB copy
 # reads 8.389e6 requests); profiles/pmc_mix_c2_learned.md
 MEASURED_L2_READ_BYTES = {"learned_tol": 6.6959e10 * 128}
@@ -584,12 +586,12 @@ def main():
         out["ihc_lambda_roofline"]["valu_issue"] = {
             "insts_per_launch": MEASURED_VALU_INSTS["ihc_lambda"],
             "frac": MEASURED_VALU_INSTS["ihc_lambda"] / (float(np.mean(ihc_ms)) * 1e-3) / VALU_ISSUE_PER_S,
-            "capacity_per_s": VALU_ISSUE_PER_S}
+            "capacity_per_s": VALU_ISSUE_PER_S, "capacity_model": VALU_ISSUE_MODEL}
     if args.signals == 256 and args.variant in MEASURED_VALU_INSTS:
         out["roofline"]["valu_issue"] = {
             "insts_per_launch": MEASURED_VALU_INSTS[args.variant],
             "frac": MEASURED_VALU_INSTS[args.variant] / (bm_avg * 1e-3) / VALU_ISSUE_PER_S,
-            "capacity_per_s": VALU_ISSUE_PER_S}
+            "capacity_per_s": VALU_ISSUE_PER_S, "capacity_model": VALU_ISSUE_MODEL}
     if args.variant == "learned_tol":
         out["roofline"]["kernel"] = "bm_ws_kernel"
         if args.signals == 256:
