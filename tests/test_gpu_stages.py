@@ -238,6 +238,12 @@ def test_jnd_chain_end_to_end(oracle):
     mr, fra, fai = oracle.jnd(lo, n_full, c["L"], c["head"], c["length"], dA, ref_idx, nIHC,
                               np.array(stages.W_DEFAULT, np.float32), FS, 0.2 * FS)
     ai_o, rate_o = stages.jnd_final(fra.reshape(-1), fai.reshape(-1), n_full, n_full, refs, serial)
+    # per-interval mean rates (getCudaMeanRate): FP64 time sums in order on both sides
+    mr_g = np.zeros(3 * n_full * 256, np.float32)
+    check(L.GeneralKernel_Copy_Results_Bytes(ptr(mr_g), L.getCudaMeanRate(), mr_g.nbytes, 0), "mean rate copy")
+    mr_g = mr_g.reshape(3, n_full, 256)
+    _say(f"mean rate rel L2 {_rel(mr_g, mr):.3e} exact {np.mean(mr_g == mr):.6f}")
+    assert _rel(mr_g, mr) <= 1e-6
     sel = calc
     _say(f"JND AI dB gpu {np.round(ai[serial[sel]], 3)} oracle {np.round(ai_o[serial[sel]], 3)}")
     assert np.max(np.abs(ai[serial[sel]] - ai_o[serial[sel]])) <= 0.1
