@@ -130,7 +130,116 @@ template <typename T> void thomas_exact(const T *Y, const T *mf, const T *low, T
   }
 }
 
+// ------------------------------------------------------ step control -----
+// Mutants: -DORC_MUTANT=k compiles one plausible transcription mistake of the
+// step-control ladder into this oracle, so that a test can show the ladder
+// pins reject it.  The default build (ORC_MUTANT 0) is the restatement.
+//   1 middle band 0.5 <= L <= 6, F <= 0 accepts instead of iterating
+//   2 ode::Tester thresholds (L > 2 halves, L < 0.25 doubles)
+//   3 F > 0 tested before L > 6
+//   4 |v| instead of signed v inside fmax(tol, v dt)
+//   5 F counts violating sections instead of summing their excess
+//   6 a halving pass re-runs the step instead of committing it
+//   7 a halving step advances local time by the old dt
+//   8 e_v >= m1tol (not strict) enables L
+//   9 e_a summed instead of maximised
+#ifndef ORC_MUTANT
+#define ORC_MUTANT 0
+#endif
+
+// Outcome of one pass's step control.
+template <typename T> struct Decision {
+  T emax, esum, fsum, L;
+  int another, reset, rerun;
+};
+
+// The Lipschitz / fault test of kernel #17 (include/cochlea.cuh:14021-14088)
+// as lane 0 computes it (Q1): the per-section terms
+//   ex = |a_ref - a|, ey = |v - v_ref|, ez = ey - max(tol, v dt) (v signed),
+// reduced to e_a = max ex, e_v = sum ey, F = sum ez by the fixed tree; then
+//   L = e_a dt / e_v if e_v > m1tol else 0, and the ladder
+//   L > 6: dt <- max(dt/2, dt_min), exit (the step commits, Q2)
+//   F > 0: iterate
+//   L < 0.5: dt <- min(2 dt, dt_max), exit
+//   else (0.5 <= L <= 6, F <= 0): iterate (no middle-band accept, Q3).
+// decision_mode 1 replaces the three reductions by zeros (what a lane
+// holding no partial sums decides).  dt is updated in place.
+template <typename T>
+Decision<T> step_control(const T *aref, const T *na, const T *vref, const T *nv, T &dt, T mtt, T mmspt, T dt_min,
+                         T dt_max, int decision_mode) {
+  T ex[NS], ey[NS], ez[NS];
+  for (int i = 0; i < NS; ++i) {
+    ex[i] = std::fabs(aref[i] - na[i]);
+    ey[i] = std::fabs(nv[i] - vref[i]);
+#if ORC_MUTANT == 4
+    ez[i] = ey[i] - std::fmax(mtt, std::fabs(nv[i]) * dt);
+#else
+    ez[i] = ey[i] - std::fmax(mtt, nv[i] * dt);
+#endif
+#if ORC_MUTANT == 5
+    ez[i] = ez[i] > 0 ? (T)1 : (T)0;
+#endif
+  }
+  Decision<T> d;
+#if ORC_MUTANT == 9
+  d.emax = tree_sum(ex);
+#else
+  d.emax = tree_max(ex);
+#endif
+  d.esum = tree_sum(ey);
+  d.fsum = tree_sum(ez);
+  d.rerun = 0;
+  if (decision_mode == 1) { d.emax = 0; d.esum = 0; d.fsum = 0; }
+  T L;
+#if ORC_MUTANT == 8
+  if (d.esum >= mmspt) { L = d.emax * dt; L = L / d.esum; } else L = 0;
+#else
+  if (d.esum > mmspt) { L = d.emax * dt; L = L / d.esum; } else L = 0;
+#endif
+  d.L = L;
+#if ORC_MUTANT == 2
+  const T hi = (T)2, lo = (T)0.25f;
+#else
+  const T hi = (T)6, lo = (T)0.5f;
+#endif
+#if ORC_MUTANT == 3
+  if (d.fsum > 0) {
+    d.another = 1; d.reset = 0;
+  } else if (L > hi) {
+#else
+  if (L > hi) {
+#endif
+    dt = dt * (T)0.5f;
+    if (dt < dt_min) dt = dt_min;
+    d.another = 0; d.reset = 1;
+#if ORC_MUTANT == 6
+    d.another = 1; d.rerun = 1;
+#endif
+#if ORC_MUTANT != 3
+  } else if (d.fsum > 0) {
+    d.another = 1; d.reset = 0;
+#endif
+  } else if (L < lo) {
+    dt = dt * (T)2;
+    if (dt > dt_max) dt = dt_max;
+    d.another = 0; d.reset = 1;
+  } else {
+#if ORC_MUTANT == 1
+    d.another = 0; d.reset = 1;
+#else
+    d.another = 1; d.reset = 0;
+#endif
+  }
+  return d;
+}
+
 // ------------------------------------------------------ kernel #17 -------
+// Optional per-pass record of one item's run (orc_bm_trace_f32).
+struct Trace {
+  orc_pass_record *rec;
+  int cap, n;
+};
+
 template <typename T> struct Outs {
   T *speeds;
   int64_t rows;
@@ -144,7 +253,7 @@ template <typename T> struct Outs {
 // One time block integrated by kernel #17 (include/cochlea.cuh:13573-14245).
 template <typename T>
 void bm_item(const orc_bm_scalars &sc, const float *sec, const float *gamma_set, const orc_bm_item &it,
-             int item_index, const float *input, int64_t input_len, const Outs<T> &o) {
+             int item_index, const float *input, int64_t input_len, const Outs<T> &o, Trace *tr = nullptr) {
   // per-section parameters (:13700-13736); promoted from the packed FP32 values
   T rsm[NS], rmass[NS], gam[NS], R[NS], S[NS], Q[NS], Sohc[NS], Stm[NS], Rtm[NS], low[NS], mf[NS];
   for (int i = 0; i < NS; ++i) {
@@ -177,7 +286,7 @@ void bm_item(const orc_bm_scalars &sc, const float *sec, const float *gamma_set,
   T *nd = &st[7 * NS], *nv = &st[8 * NS], *na = &st[9 * NS], *npsi = &st[10 * NS], *npsid = &st[11 * NS];
   T *nxis = &st[12 * NS], *nxi = &st[13 * NS], *Ptm = &st[14 * NS];
   T *p = &st[15 * NS], *pn = &st[16 * NS], *Y = &st[17 * NS], *G = &st[18 * NS], *od = &st[19 * NS];
-  T *vref = &st[20 * NS], *aref = &st[21 * NS], *ex = &st[22 * NS], *ey = &st[23 * NS], *ez = &st[24 * NS];
+  T *vref = &st[20 * NS], *aref = &st[21 * NS];
   T *pold = &st[26 * NS], *pext = &st[27 * NS];   // previous step's pressure; extrapolated guess
   T ow_pd = 0, ow_pv = 0, ow_pa = 0, ow_nd = 0, ow_nv = 0, ow_na = 0;
 
@@ -194,7 +303,12 @@ void bm_item(const orc_bm_scalars &sc, const float *sec, const float *gamma_set,
   const float span = (float)(it.mesi - it.mosi);
 
   while (in_s < it.mesi) {
+    if (tr && tr->n >= tr->cap) break;       // trace full: stop the run
     int out_loop = 0, reset = 1, another = 1;
+    T dt_commit = dt;
+#if ORC_MUTANT == 6
+    int reruns = 0;
+#endif
     while (another) {
       for (int i = 0; i < NS; ++i) { vref[i] = nv[i]; aref[i] = na[i]; }
       if (out_loop == 0 || reset) {        // Euler predictor (:13770-13795)
@@ -308,34 +422,37 @@ void bm_item(const orc_bm_scalars &sc, const float *sec, const float *gamma_set,
       for (int i = 0; i < NS; ++i)          // psi derivative (:14011)
         npsid[i] = (eta1 * od[i] + eta2 * (nxis[i] - nv[i])) + (-w_ohc) * npsi[i];
 
-      if (reset == 0) {                      // step control (:14021-14088)
-        for (int i = 0; i < NS; ++i) {
-          ex[i] = std::fabs(aref[i] - na[i]);
-          ey[i] = std::fabs(nv[i] - vref[i]);
-          ez[i] = ey[i] - std::fmax(mtt, nv[i] * dt);
-        }
-        T emax = tree_max(ex), esum = tree_sum(ey), fsum = tree_sum(ez);
-        if (sc.decision_mode == 1) { emax = 0; esum = 0; fsum = 0; }
-        T L;
-        if (esum > mmspt) { L = emax * dt; L = L / esum; } else L = 0;
-        if (L > (T)6) {
-          dt = dt * half;
-          if (dt < dt_min) dt = dt_min;
-          another = 0; reset = 1;
-        } else if (fsum > 0) {
-          another = 1; reset = 0;
-        } else if (L < half) {
-          dt = dt * (T)2;
-          if (dt > dt_max) dt = dt_max;
-          another = 0; reset = 1;
-        } else {
-          another = 1; reset = 0;
-        }
-      } else {
+      const T dt_in = dt;
+      const int decided = reset == 0;
+      Decision<T> d{0, 0, 0, 0, 1, 0, 0};
+      if (decided) {                         // step control (:14021-14088)
+        d = step_control<T>(aref, na, vref, nv, dt, mtt, mmspt, dt_min, dt_max, sc.decision_mode);
+        another = d.another; reset = d.reset;
+      } else {                               // pass 0 always iterates (:14095-14104)
         another = 1; reset = 0;
       }
+      const int pass = out_loop;
       out_loop++;
-      if (out_loop >= sc.max_passes) { another = 0; reset = 1; c_fail = 1.f; }
+      int capped = 0;
+      if (out_loop >= sc.max_passes) { another = 0; reset = 1; c_fail = 1.f; capped = 1; }
+#if ORC_MUTANT == 6
+      if (d.rerun && (capped || ++reruns >= 64)) another = 0;
+#endif
+      if (tr && tr->n < tr->cap) {
+        orc_pass_record &r = tr->rec[tr->n++];
+        r.step = (int32_t)n_steps; r.pass = pass; r.in_s = in_s; r.decided = decided;
+        r.t = (float)t; r.dt_in = (float)dt_in; r.dt_out = (float)dt;
+        r.L = (float)d.L; r.F = (float)d.fsum; r.e_a = (float)d.emax; r.e_v = (float)d.esum;
+        r.another = another; r.reset = reset; r.capped = capped;
+      }
+#if ORC_MUTANT == 6
+      if (another) continue;   // restart the step from the predictor with the halved dt
+#endif
+#if ORC_MUTANT == 7
+      dt_commit = dt_in;
+#else
+      dt_commit = dt;
+#endif
     }
     // committed step counters (:14130-14141)
     c_steps += 1.0f;
@@ -370,7 +487,7 @@ void bm_item(const orc_bm_scalars &sc, const float *sec, const float *gamma_set,
       ppsid[i] = npsid[i]; pxis[i] = nxis[i]; pxi[i] = nxi[i];
     }
     ow_pd = ow_nd; ow_pv = ow_nv; ow_pa = ow_na;
-    t += dt;
+    t += dt_commit;   // the new dt, also after a halving pass (Q2)
   }
   if (o.failed_blocks) o.failed_blocks[item_index] = fb;
   if (o.conv_time_blocks) o.conv_time_blocks[item_index] = ctb;
@@ -515,6 +632,32 @@ int orc_bm_run_f64(const orc_bm_scalars *sc, const float *sec, const float *gamm
   Outs<double> o{speeds, rows, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
                  item_steps, item_passes, item_sweeps};
   return bm_run<double>(sc, sec, gamma, items, n_items, input, input_len, o, n_threads);
+}
+
+int orc_step_control_f32(const float *aref, const float *na, const float *vref, const float *nv, float dt, float tol,
+                         float m1tol, float dt_min, float dt_max, int decision_mode, float *out5,
+                         int32_t *another_reset) {
+  if (!aref || !na || !vref || !nv || !out5 || !another_reset) return -1;
+  const Decision<float> d = step_control<float>(aref, na, vref, nv, dt, tol, m1tol, dt_min, dt_max, decision_mode);
+  out5[0] = dt;
+  out5[1] = d.L;
+  out5[2] = d.emax;
+  out5[3] = d.esum;
+  out5[4] = d.fsum;
+  another_reset[0] = d.another;
+  another_reset[1] = d.reset;
+  return 0;
+}
+
+int orc_bm_trace_f32(const orc_bm_scalars *sc, const float *sec, const float *gamma, const orc_bm_item *item,
+                     const float *input, int64_t input_len, orc_pass_record *rec, int cap, int *n_rec) {
+  if (!sc || !sec || !gamma || !item || !input || !rec || cap <= 0 || !n_rec) return -1;
+  Outs<float> o{nullptr, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                nullptr, nullptr, nullptr};
+  Trace tr{rec, cap, 0};
+  bm_item<float>(*sc, sec, gamma + (int64_t)item->set * NS, *item, 0, input, input_len, o, &tr);
+  *n_rec = tr.n;
+  return 0;
 }
 
 int orc_jacobi(const float *Y, const float *rsm, float *p, int sweeps) {

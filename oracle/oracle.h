@@ -93,6 +93,30 @@ int orc_bm_run_f64(const orc_bm_scalars *sc, const float *sec_f32, const float *
                    double *speeds, int64_t rows, int64_t *item_steps, int64_t *item_passes,
                    int64_t *item_sweeps, int n_threads);
 
+/* Step control of one pass (include/cochlea.cuh:14021-14088, lane-0 decision):
+ * per-section vectors of 256 floats; dt in, then out5 = {dt after the pass,
+ * L, e_a = max|a_ref - a|, e_v = sum|v - v_ref|, F = sum(|v - v_ref| - max(tol, v dt))}
+ * and another_reset = {another_loop, reset_iteration}. */
+int orc_step_control_f32(const float *aref, const float *na, const float *vref, const float *nv, float dt, float tol,
+                         float m1tol, float dt_min, float dt_max, int decision_mode, float *out5,
+                         int32_t *another_reset);
+
+/* One record per predictor/corrector pass of a traced item. */
+typedef struct {
+  int32_t step;      /* committed steps before this pass's step */
+  int32_t pass;      /* pass index within the step (0 = predictor) */
+  int32_t in_s;      /* input sample at the pass */
+  int32_t decided;   /* 1 when step control ran (every pass after the predictor) */
+  float t;           /* local time at the pass */
+  float dt_in, dt_out;
+  float L, F, e_a, e_v;
+  int32_t another, reset, capped;   /* after the decision and the pass cap */
+} orc_pass_record;
+/* Kernel-17 run of one item that records its passes (up to cap; the run stops
+ * once the record is full). */
+int orc_bm_trace_f32(const orc_bm_scalars *sc, const float *sec, const float *gamma, const orc_bm_item *item,
+                     const float *input, int64_t input_len, orc_pass_record *rec, int cap, int *n_rec);
+
 /* Jacobi sweeps of the kernel convention (Q7) on one right-hand side. */
 int orc_jacobi(const float *Y, const float *rsm, float *p, int sweeps);
 

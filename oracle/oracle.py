@@ -13,7 +13,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# CM_ORACLE_LIB points the tests at another build of the same sources (the
+# mutant builds of the step-control pins); the default is the in-tree build.
+_LIB_PATH = os.environ.get("CM_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 _lib = None
 
 SECTIONS = 256
@@ -150,6 +152,39 @@ def bm_run(sc: BMScalars, sec: np.ndarray, gamma: np.ndarray, items: np.ndarray,
         out.update(speeds=speeds)
     assert rc == 0, rc
     return out
+
+
+PASS_RECORD_DTYPE = np.dtype([("step", np.int32), ("pass_", np.int32), ("in_s", np.int32), ("decided", np.int32),
+                              ("t", np.float32), ("dt_in", np.float32), ("dt_out", np.float32), ("L", np.float32),
+                              ("F", np.float32), ("e_a", np.float32), ("e_v", np.float32), ("another", np.int32),
+                              ("reset", np.int32), ("capped", np.int32)])
+
+
+def step_control(aref, na, vref, nv, dt: float, tol: float, m1tol: float, dt_min: float = 1e-18,
+                 dt_max: float = 1e-6, decision_mode: int = 0) -> dict:
+    """One pass's step control (include/cochlea.cuh:14021-14088) on 256-section vectors."""
+    vecs = [np.ascontiguousarray(np.broadcast_to(np.asarray(x, np.float32), (SECTIONS,))) for x in (aref, na, vref, nv)]
+    out = np.zeros(5, np.float32)
+    ar = np.zeros(2, np.int32)
+    rc = lib().orc_step_control_f32(*[_p(v) for v in vecs], C.c_float(dt), C.c_float(tol), C.c_float(m1tol),
+                                    C.c_float(dt_min), C.c_float(dt_max), C.c_int(decision_mode), _p(out), _p(ar))
+    assert rc == 0
+    return {"dt": out[0], "L": out[1], "e_a": out[2], "e_v": out[3], "F": out[4], "another": int(ar[0]),
+            "reset": int(ar[1])}
+
+
+def bm_trace(sc: BMScalars, sec: np.ndarray, gamma: np.ndarray, item: np.ndarray, inp: np.ndarray,
+             cap: int) -> np.ndarray:
+    """Per-pass records of one item's kernel-17 run (stops after cap passes)."""
+    rec = np.zeros(cap, PASS_RECORD_DTYPE)
+    n = C.c_int(0)
+    item = np.ascontiguousarray(np.atleast_1d(item)[:1], ITEM_DTYPE)
+    inp = np.ascontiguousarray(inp, np.float32)
+    rc = lib().orc_bm_trace_f32(C.byref(sc), _p(np.ascontiguousarray(sec, np.float32)),
+                                _p(np.ascontiguousarray(np.atleast_2d(gamma), np.float32)), _p(item), _p(inp),
+                                C.c_int64(inp.size), _p(rec), C.c_int(cap), C.byref(n))
+    assert rc == 0
+    return rec[:n.value]
 
 
 def jacobi(Y: np.ndarray, rsm: np.ndarray, p0: np.ndarray, sweeps: int) -> np.ndarray:
