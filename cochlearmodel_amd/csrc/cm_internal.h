@@ -91,6 +91,13 @@ struct JndArgs {
   float *mean_rate;         // 3 x n_intervals x 256
   float *F_RA;
   float *FisherAISum;
+  // GlobalCalculateJND's backup stages (src/cochlea.cu:3860-3918) written to
+  // buffer when backup_stage is 1..6 (NULL / 0: none): 1 AvgMeanRate
+  // [3][n_intervals]; 2 dMeanRate, 5 preFisherAITimeReduced, 6 CRLB_RA
+  // [3][n_intervals][256]; 3 dLambda, 4 preFisherAI at the lambda nodes of
+  // the interval bodies ([3][rows][256])
+  int backup_stage;
+  float *buffer;
 };
 
 void set_error(const char *fmt, ...);

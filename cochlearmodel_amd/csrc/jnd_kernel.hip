@@ -40,6 +40,10 @@ __global__ __launch_bounds__(256) void jnd_kernel(JndArgs a) {
   const int64_t c = ((int64_t)li * a.rows + (int64_t)iv * a.interval_nodes + a.head_nodes) * CM_SECTIONS + s;
   const int64_t r = ((int64_t)li * a.rows + (int64_t)ref * a.interval_nodes + a.head_nodes) * CM_SECTIONS + s;
   const int L = a.length_nodes;
+  const int stage = a.buffer ? a.backup_stage : 0;
+  // per-node stages 3 (dLambda) and 4 (preFisherAI) at the body's own lambda nodes
+  float *node_stage = (stage == 3 || stage == 4) ? a.buffer + c : nullptr;
+  const int64_t ms = ((int64_t)li * a.n_intervals + iv) * CM_SECTIONS + s;   // mean-rate layout
   float ai = 0.f, rms = 0.f;
   if (a.calc_ai || a.calc_rms) {
     double acc_c = a.lambda[c], acc_r = a.lambda[r];
@@ -47,7 +51,9 @@ __global__ __launch_bounds__(256) void jnd_kernel(JndArgs a) {
     if (a.calc_ai) {
       const float refl = a.lambda[r], cal = a.lambda[c];
       const float dl = dAv > 0 ? (cal - refl) / dAv : 0.0f;
-      accv += (float)((double)(dl * dl) / (a.Fs * (double)refl));
+      const float pre = (float)((double)(dl * dl) / (a.Fs * (double)refl));
+      accv += pre;
+      if (node_stage) node_stage[0] = stage == 3 ? dl : pre;
 #pragma unroll 4
       for (int t = 1; t < L; ++t) {
         const float refl = a.lambda[r + (int64_t)t * CM_SECTIONS];
@@ -55,7 +61,9 @@ __global__ __launch_bounds__(256) void jnd_kernel(JndArgs a) {
         acc_c = acc_c + cal;
         acc_r = acc_r + refl;
         const float dl = dAv > 0 ? (cal - refl) / dAv : 0.0f;
-        accv += (float)((double)(dl * dl) / (a.Fs * (double)refl));
+        const float pre = (float)((double)(dl * dl) / (a.Fs * (double)refl));
+        accv += pre;
+        if (node_stage) node_stage[(int64_t)t * CM_SECTIONS] = stage == 3 ? dl : pre;
       }
     } else {
 #pragma unroll 8
@@ -66,15 +74,22 @@ __global__ __launch_bounds__(256) void jnd_kernel(JndArgs a) {
     }
     const float mr = (float)acc_c / (float)L;
     const float mref = (float)acc_r / (float)L;
-    a.mean_rate[((int64_t)li * a.n_intervals + iv) * CM_SECTIONS + s] = mr;
+    a.mean_rate[ms] = mr;
+    const float dMR = (mr - mref) / dAv;
+    if (stage == 1 && a.calc_rms) {   // mean over the sections of this interval's mean rate
+      const float sum = halving_tree(sh, mr, s);
+      if (s == 0) a.buffer[(int64_t)li * a.n_intervals + iv] = sum / (float)CM_SECTIONS;
+    }
+    if (stage == 2) a.buffer[ms] = dMR;
     if (a.calc_ai) {
       const float rs = 1.0f / __builtin_sqrtf(accv);
       ai = (float)((double)(a.T / a.Tlength) * a.nIHC[s] / (double)(rs * rs));
+      if (stage == 5) a.buffer[ms] = ai;
     }
     if (a.calc_rms) {
-      const float dMR = (mr - mref) / dAv;
       const float crlb = 1.0f / __builtin_sqrtf(((a.T / mref) * dMR) * dMR);
       rms = (float)(a.nIHC[s] / (double)(crlb * crlb));
+      if (stage == 6) a.buffer[ms] = crlb;
     }
   }
   const int64_t fi = (int64_t)li * a.out_stride + a.out_index[iv];

@@ -346,6 +346,13 @@ int cm_ihc_lambda_run(const float *d_bm, int64_t rows, const cm_ihc_interval *h_
     set_error("cm_ihc_lambda_run: IIR filter needs its interval table");
     return -3;
   }
+  // the IIR pass runs each of its intervals from rest with period len (the
+  // kbase / L / tla / frdp fields are not read): rows [row0, row0 + len)
+  for (int i = 0; d_ac_a && i < n_iir_ivs; ++i)
+    if (h_iir_ivs[i].row0 < 0 || h_iir_ivs[i].len < 0 || h_iir_ivs[i].row0 + h_iir_ivs[i].len > rows) {
+      set_error("cm_ihc_lambda_run: IIR interval %d outside [0, %lld)", i, (long long)rows);
+      return -3;
+    }
   if (d_ac_a && g_st.iir_ivs.upload(h_iir_ivs, n_iir_ivs)) return -1;
   if (pad_taps(g_st.ac_bpad, d_ac_b, n_ac) || pad_taps(g_st.dcpad, d_dc, n_dc)) return -1;
   const int rc = cm_launch_ihc_lambda(d_bm, rows, g_st.ivs.ptr, h_ivs, n_ivs, g_st.iir_ivs.ptr, n_iir_ivs, d_ac_b,
@@ -357,8 +364,9 @@ int cm_ihc_lambda_run(const float *d_bm, int64_t rows, const cm_ihc_interval *h_
 
 // ---------------------------------------------------------------- JND ---
 // Local interval -> reference interval of GlobalCalculateJND (:3828-3855).
+// serial[g] of a calculated interval indexes to_ref (n_calc entries).
 static int reference_map(int JNDIntervals, int handled, const int *refs, int nrefs, const int *serial,
-                         const int *to_ref, std::vector<int32_t> &out) {
+                         const int *to_ref, int n_calc, std::vector<int32_t> &out) {
   out.resize(JNDIntervals);
   for (int i = 0; i < JNDIntervals; ++i) {
     const int g = i + handled;
@@ -366,7 +374,14 @@ static int reference_map(int JNDIntervals, int handled, const int *refs, int nre
     for (int r = 0; r < nrefs; ++r)
       if (refs[r] == g) is_ref = true;
     int gref = g;
-    if (!is_ref && nrefs > 0) gref = to_ref[serial[g]];
+    if (!is_ref && nrefs > 0) {
+      if (serial[g] < 0 || serial[g] >= n_calc) {
+        set_error("CudaCalculateJND: serial position %d of interval %d outside %d calculated intervals", serial[g],
+                  g, n_calc);
+        return -3;
+      }
+      gref = to_ref[serial[g]];
+    }
     const int local = gref - handled;
     if (local < 0 || local >= JNDIntervals) {
       set_error("CudaCalculateJND: interval %d maps to reference %d outside this run", g, gref);
@@ -388,26 +403,71 @@ int CudaCalculateJND(int calculate_ai, int calculate_rms, int mean_size, int fis
                      double JND_Delta_Alpha_Length_Factor, int JND_USE_Spont_Base_Reference_Lambda,
                      int Show_Run_Time, int calcdA, int show_generated_input_params_cuda, int backup_stage) {
   (void)mean_size; (void)SPLRefVal; (void)scaleBMVelocityForLambdaCalculation; (void)JND_Calculated_Intervals;
-  (void)numOFJNDCalculated; (void)writeMatrixSize; (void)JND_USE_Spont_Base_Reference_Lambda; (void)Show_Run_Time;
-  (void)show_generated_input_params_cuda; (void)backup_stage;
+  (void)writeMatrixSize; (void)JND_USE_Spont_Base_Reference_Lambda; (void)Show_Run_Time;
+  (void)show_generated_input_params_cuda;
   if (JNDIntervals <= 0) return 0;
   if (!g_st.lambda.ptr || calculatedMatrixSize <= 0 || calculatedMatrixSize % CM_SECTIONS) {
     set_error("CudaCalculateJND: no lambda buffer / bad calculatedMatrixSize");
     return -3;
   }
   const int64_t rows = calculatedMatrixSize / CM_SECTIONS;
-  if ((int64_t)JNDIntervals * JNDIntervalNodes > rows) {
+  // the lambda planes have a stride of `rows` rows: all three must lie in the
+  // buffer RunIHCKernel filled
+  if ((size_t)3 * rows * CM_SECTIONS > g_st.lambda.cap) {
+    set_error("CudaCalculateJND: 3 planes of %lld rows exceed the lambda buffer (%zu nodes)", (long long)rows,
+              g_st.lambda.cap);
+    return -3;
+  }
+  if (JNDIntervalNodes <= 0 || (int64_t)JNDIntervals * JNDIntervalNodes > rows) {
     set_error("CudaCalculateJND: %d intervals of %d nodes exceed %lld rows", JNDIntervals, JNDIntervalNodes,
               (long long)rows);
+    return -3;
+  }
+  // each interval body [head, head + length) must lie inside its interval
+  if (JNDIntervalHeadNodes < 0 || lengthOffset <= 0 || JNDIntervalHeadNodes + lengthOffset > JNDIntervalNodes) {
+    set_error("CudaCalculateJND: body [%d, %d) outside an interval of %d nodes", JNDIntervalHeadNodes,
+              JNDIntervalHeadNodes + lengthOffset, JNDIntervalNodes);
+    return -3;
+  }
+  // this run's columns handled .. handled + JNDIntervals - 1 of the Fisher arrays
+  if (handeledIntervalsJND < 0 || JNDIntervalsFull < handeledIntervalsJND + JNDIntervals) {
+    set_error("CudaCalculateJND: intervals %d + %d exceed JNDIntervalsFull %d", handeledIntervalsJND, JNDIntervals,
+              JNDIntervalsFull);
     return -3;
   }
   if (fisher_size < 3 * JNDIntervalsFull) {
     set_error("CudaCalculateJND: fisher_size %d < 3 x %d", fisher_size, JNDIntervalsFull);
     return -3;
   }
+  if (numOFJNDReferences < 0 || (numOFJNDReferences > 0 && (!JND_Refrence_Intervals ||
+                                                             !JND_Serial_Intervals_Positions ||
+                                                             !JND_Interval_To_Reference))) {
+    set_error("CudaCalculateJND: reference tables missing");
+    return -3;
+  }
+  if (calcdA && (size_t)overlapNodes + (size_t)(JNDIntervals - 1) * JNDIntervalNodes + JNDIntervalHeadNodes +
+                        lengthOffset > bm_input_cap()) {
+    set_error("CudaCalculateJND: dA reads past the BM input buffer");
+    return -3;
+  }
+  if (overlapNodes < 0) {
+    set_error("CudaCalculateJND: overlapNodes < 0");
+    return -3;
+  }
+  // stage dumps go to the copy buffer (getCudaBuffer), sized for the stage
+  size_t stage_nodes = 0;
+  if (backup_stage == 1) stage_nodes = (size_t)3 * JNDIntervals;
+  else if (backup_stage == 2 || backup_stage == 5 || backup_stage == 6)
+    stage_nodes = (size_t)3 * JNDIntervals * CM_SECTIONS;
+  else if (backup_stage == 3 || backup_stage == 4)
+    stage_nodes = (size_t)3 * rows * CM_SECTIONS;
+  if (stage_nodes > g_st.buffer1.cap) {
+    if (g_st.buffer1.ensure(stage_nodes)) return -1;
+    g_st.buffer_valid = 0;   // a new buffer holds no saved lambda
+  }
   std::vector<int32_t> ref;
   int rc = reference_map(JNDIntervals, handeledIntervalsJND, JND_Refrence_Intervals, numOFJNDReferences,
-                         JND_Serial_Intervals_Positions, JND_Interval_To_Reference, ref);
+                         JND_Serial_Intervals_Positions, JND_Interval_To_Reference, numOFJNDCalculated, ref);
   if (rc) return rc;
   std::vector<int32_t> outi(JNDIntervals);
   for (int i = 0; i < JNDIntervals; ++i) outi[i] = i + handeledIntervalsJND;
@@ -450,6 +510,8 @@ int CudaCalculateJND(int calculate_ai, int calculate_rms, int mean_size, int fis
   a.mean_rate = g_st.mean_rate.ptr;
   a.F_RA = g_st.F_RA.ptr;
   a.FisherAISum = g_st.FAI.ptr;
+  a.backup_stage = stage_nodes ? backup_stage : 0;
+  a.buffer = stage_nodes ? g_st.buffer1.ptr : nullptr;
   if (cm_launch_jnd(&a, stream())) {
     set_error("CudaCalculateJND: launch failed");
     return -1;
@@ -509,6 +571,8 @@ int cm_jnd_run(const float *d_lambda, int64_t rows, int n_intervals, int interva
   a.mean_rate = d_mean_rate;
   a.F_RA = d_F_RA;
   a.FisherAISum = d_FisherAISum;
+  a.backup_stage = 0;
+  a.buffer = nullptr;
   return cm_launch_jnd(&a, stream()) ? -1 : 0;
 }
 

@@ -42,6 +42,14 @@
 // Complex_JND_Calculation_Types, Approximated_JND_EPS_Diff, Max_dB_SPL_JND, W,
 // Input_Noise / Input_Noise_File, JND_File_Name, JND_Raw_File_Name.  Noise_Seed
 // (no reference counterpart) seeds the white noise when none is given.
+// TEST_File_Target with Type_TEST_Output (default MeanRate): one stage written
+// as raw float32 -- the IHC stages AC, DC, dS, Shigh, IHC, PRE_IHC, the lambda
+// planes JND_Lambda[k], and with Calculate_JND the JND stages MeanRate,
+// AvgMeanRate, dMeanRate, dLambda[k], preFisherAI, preFisherAITimeReduced and
+// CRLB_RA (src/AudioGramCreator.cpp:884-971).  Where the reference writes a
+// stale host vector for AvgMeanRate, preFisherAI and preFisherAITimeReduced
+// (no branch of :938-964 copies them), the stage the kernel computed is
+// written.
 // Keys of this library's solver options (no reference counterpart):
 // Jacobi_Tolerance (> 0: Jacobi to that relative residual instead of the fixed
 // sweeps), Jacobi_Check_Every (4), Jacobi_Max_Sweeps (1000), Warm_Start (0
@@ -258,6 +266,55 @@ std::string dir_of(const std::string &p) {
   return s == std::string::npos ? "." : p.substr(0, s);
 }
 
+// Stage dumps (TEST_File_Target, Type_TEST_Output; src/params.cpp:109, 165-187,
+// 1695-1707): the backup stage number of a Type_TEST_Output name.  1..6 are
+// stages of the JND kernel, 9..14 of the IHC chain; 0 names a stage read
+// without a backup (JND_Lambda[k], MeanRate).  Unknown names are refused.
+struct TestTarget {
+  std::string path, type;
+  int stage = -1;      // -1: no TEST_File_Target
+  int plane = -1;      // dLambdaK / JND_LambdaK: lambda plane k, else all three
+};
+
+TestTarget test_target(const ParFile &par) {
+  TestTarget t;
+  if (!par.has("TEST_File_Target")) return t;
+  t.path = par.str("TEST_File_Target");
+  t.type = par.str("Type_TEST_Output");
+  if (t.type.empty()) t.type = "MeanRate";
+  static const std::map<std::string, int> kStages = {
+      {"AvgMeanRate", 1}, {"dMeanRate", 2}, {"dLambda", 3}, {"preFisherAI", 4}, {"preFisherAITimeReduced", 5},
+      {"CRLB_RA", 6}, {"AC", 9}, {"DC", 10}, {"dS", 11}, {"Shigh", 12}, {"IHC", 13}, {"PRE_IHC", 14},
+      {"JND_Lambda", 0}, {"MeanRate", 0}};
+  std::string base = t.type;
+  // the reference matches the per-plane forms as dLambda<digit> / JND_Lambda<digit>
+  // (src/AudioGramCreator.cpp:925, 942); "dLambda(k)" is accepted as well
+  for (const char *pre : {"dLambda", "JND_Lambda"}) {
+    const size_t n = std::strlen(pre);
+    if (t.type.size() > n && t.type.compare(0, n, pre) == 0) {
+      std::string rest = t.type.substr(n);
+      if (rest.size() == 3 && rest[0] == '(' && rest[2] == ')') rest = rest.substr(1, 1);
+      if (rest.size() == 1 && std::isdigit((unsigned char)rest[0])) {
+        t.plane = rest[0] - '0';
+        if (t.plane >= kLambdas) throw std::runtime_error("index of lambda must be smaller than 3, you chose " + rest);
+        base = pre;
+      }
+    }
+  }
+  const auto it = kStages.find(base);
+  if (it == kStages.end()) throw std::runtime_error("unknown Type_TEST_Output " + t.type);
+  t.stage = it->second;
+  return t;
+}
+
+// saveArrayToDisk without the time column (src/AudioGramCreator.cpp:25-60):
+// raw float32
+void write_test_target(const TestTarget &t, const float *v, size_t n) {
+  BinWriter w(t.path);
+  w.write(v, n);
+  std::printf("cochlea_run: %s (%zu values) written to %s\n", t.type.c_str(), n, t.path.c_str());
+}
+
 // ------------------------------------------------------------------ run ---
 int run(const ParFile &par, const std::string &exe_dir) {
   const float Fs = par.scalar<float>("Fs", 44100.f);
@@ -272,6 +329,9 @@ int run(const ParFile &par, const std::string &exe_dir) {
   // Run_Stage_Vector
   const int run_stage = par.scalar<int>("Run_Stage_Calculation", 0);
   if (run_stage < 0 || run_stage > 2) throw std::runtime_error("Run_Stage_Calculation must be 0, 1 or 2");
+  const TestTarget tt = test_target(par);
+  if ((tt.stage >= 1 && tt.stage <= 6) || tt.type == "MeanRate")
+    throw std::runtime_error("Type_TEST_Output " + tt.type + " is a JND stage: it needs Calculate_JND");
   if (run_stage > 0 && !par.has("Run_Stage_Vector") && par.str("Run_Stage_File_name").empty())
     throw std::runtime_error("Chosen Run stage calculation (=" + std::to_string(run_stage) + ") need data or file");
 
@@ -427,9 +487,13 @@ int run(const ParFile &par, const std::string &exe_dir) {
     for (int j = 0; j < kLambdas; ++j)
       for (int i = 0; i < kSections; ++i) aihc[j * kSections + i] = A3[j];
     ok(loadAihc(aihc.data()), "loadAihc");
+    std::vector<float> backup;
     const auto t0 = std::chrono::steady_clock::now();
     if (run_stage < 2) {
-      ok(RunIHCKernel(nullptr, 0, 1, 0, 0), "RunIHCKernel");
+      if (tt.stage >= 9) backup.resize((size_t)total * kSections);
+      ok(RunIHCKernel(backup.empty() ? nullptr : backup.data(), 0, 1, tt.stage >= 9 ? tt.stage : 0, 0),
+         "RunIHCKernel");
+      if (tt.stage >= 9) write_test_target(tt, backup.data(), backup.size());
     } else {
       // resume from lambda (src/AudioGramCreator.cpp:895-902): plane j of the
       // stage data is rows [j*total, (j+1)*total); upload, then the
@@ -450,6 +514,15 @@ int run(const ParFile &par, const std::string &exe_dir) {
     for (int j = 0; j < kLambdas; ++j)
       if (allowed & (2u << j))
         out.write(lam.data() + ((size_t)j * total + overlap) * kSections, rows * kSections);
+    if (tt.stage == 0) {
+      // JND_Lambda[k] (src/AudioGramCreator.cpp:921-934): the lambda planes
+      const size_t plane = (size_t)total * kSections;
+      std::vector<float> v(tt.plane < 0 ? kLambdas * plane : plane);
+      ok(GeneralKernel_Copy_Results_Bytes(v.data(), getCudaLambda(), v.size() * sizeof(float),
+                                          (tt.plane < 0 ? 0 : tt.plane) * plane * sizeof(float)),
+         "JND_Lambda copy");
+      write_test_target(tt, v.data(), v.size());
+    }
   }
   // timers_marked: this host's stage times in ms (BM kernel, IHC/lambda)
   const float timers[2] = {bm_ms, ihc_ms};
@@ -598,6 +671,9 @@ int run_jnd(const ParFile &par, const std::string &exe_dir) {
                                    par.vec<double>("Complex_Profile_Noise_Level_Fix_Addition"), calculate_jnd);
   const std::string jnd_file = par.str("JND_File_Name");
   if (calculate_jnd && jnd_file.empty()) throw std::runtime_error("JND File is missing, cannot output results...");
+  const TestTarget tt = test_target(par);
+  if (!calculate_jnd && ((tt.stage >= 1 && tt.stage <= 6) || tt.type == "MeanRate"))
+    throw std::runtime_error("Type_TEST_Output " + tt.type + " is a JND stage: it needs Calculate_JND");
   const std::string out_path = par.str("Output_Result");
   unsigned allowed = 0;
   if (!par.scalar<int>("Discard_BM_Velocity_Output", 0)) allowed |= 1;
@@ -735,8 +811,10 @@ int run_jnd(const ParFile &par, const std::string &exe_dir) {
   for (int j = 0; j < kLambdas; ++j)
     for (int i = 0; i < kSections; ++i) aihc[j * kSections + i] = A3[j];
   ok(loadAihc(aihc.data()), "loadAihc");
+  std::vector<float> backup;
+  if (tt.stage >= 9) backup.resize((size_t)rows * kSections);
   const auto t_ihc = std::chrono::steady_clock::now();
-  ok(RunIHCKernel(nullptr, 0, 0, 0, 0), "RunIHCKernel");
+  ok(RunIHCKernel(backup.empty() ? nullptr : backup.data(), 0, 0, tt.stage >= 9 ? tt.stage : 0, 0), "RunIHCKernel");
   ok(cm_device_synchronize(), "cm_device_synchronize");
   const float ihc_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t_ihc).count();
   if (out) {
@@ -749,7 +827,28 @@ int run_jnd(const ParFile &par, const std::string &exe_dir) {
     const float timers[2] = {bm_ms, ihc_ms};
     out->write(timers, 2);
   }
+  auto dump_stage = [&]() {
+    if (tt.stage < 0) return;
+    // the stage dumps of src/AudioGramCreator.cpp:913-966: mean-rate layout
+    // [3][intervals][256] (getMeanNodes), lambda layout [3][rows][256]
+    // (matrixSize per plane); the IHC stages as RunIHCKernel backed them up
+    const size_t plane = (size_t)rows * kSections, mean_nodes = (size_t)kLambdas * n_prof * kSections;
+    std::vector<float> v;
+    auto copy = [&](void *src, size_t n, size_t off) {
+      v.resize(n);
+      ok(GeneralKernel_Copy_Results_Bytes(v.data(), src, n * sizeof(float), off * sizeof(float)), "stage copy");
+    };
+    if (tt.stage >= 9) v = backup;
+    else if (tt.type == "MeanRate") copy(getCudaMeanRate(), mean_nodes, 0);
+    else if (tt.stage == 0) copy(getCudaLambda(), tt.plane < 0 ? kLambdas * plane : plane, tt.plane < 0 ? 0 : tt.plane * plane);
+    else if (tt.stage == 1) copy(getCudaBuffer(), (size_t)kLambdas * n_prof, 0);
+    else if (tt.stage == 3 || tt.stage == 4)
+      copy(getCudaBuffer(), tt.plane < 0 ? kLambdas * plane : plane, tt.plane < 0 ? 0 : tt.plane * plane);
+    else copy(getCudaBuffer(), mean_nodes, 0);
+    write_test_target(tt, v.data(), v.size());
+  };
   if (!calculate_jnd) {
+    dump_stage();
     std::printf("cochlea_run: profile run, %d intervals written to %s (BM %.2f ms)\n", n_prof, out_path.c_str(),
                 bm_ms);
     ok(BMOHCKernel_Free(), "BMOHCKernel_Free");
@@ -763,8 +862,10 @@ int run_jnd(const ParFile &par, const std::string &exe_dir) {
   ok(CudaCalculateJND(calc_ai, calc_rms, 0, 3 * n_prof, spl_ref, Fs, 1.0, nIHC.data(), lay.calc.data(), n_calc,
                       lay.refs.data(), (int)lay.refs.size(), 0, n_prof, n_prof, head, overlap, interval, length,
                       lay.serial.data(), lay.to_ref.data(), F_RA.data(), FAI.data(), rows * kSections,
-                      rows * kSections, dalpha == 0 ? (double)length : dalpha * Fs, 1, 0, 0, 0, 0),
+                      rows * kSections, dalpha == 0 ? (double)length : dalpha * Fs, 1, 0, 0, 0,
+                      tt.stage >= 1 && tt.stage <= 6 ? tt.stage : 0),
      "CudaCalculateJND");
+  dump_stage();
   std::vector<double> ai(n_prof, 0.0), rate(n_prof, 0.0);
   ok(cm_jnd_final(F_RA.data(), FAI.data(), n_prof, n_prof, 0, lay.refs.data(), (int)lay.refs.size(),
                   lay.serial.data(), spl_ref, par.scalar<double>("Max_dB_SPL_JND", 150.0), calc_ai, calc_rms,

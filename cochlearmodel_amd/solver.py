@@ -153,8 +153,10 @@ class BatchedBM:
 
 def run_bm_voice(signal: np.ndarray, layout: VoiceLayout, ohc=0.5, max_m1_sp_error=1e-15, max_tolerance=1e-10,
                  relative_error=0, m1_fix=0.0, tol_fix=0.0, spl_ref=2e-5, j_fast=2, j_slow=20, max_passes=10,
-                 decouple_filter=0, timed=False):
-    """Reference call sequence for one VOICE chunk; returns speeds [bufP1, 256] and diagnostics."""
+                 decouple_filter=0, timed=False, boundary_conditions=1):
+    """Reference call sequence for one VOICE chunk; returns speeds [bufP1, 256] and diagnostics.
+    boundary_conditions 0 takes the block tolerances from a preceding
+    setupToleranceProfile instead of computing them from the input."""
     L = lib()
     sec = section_params()
     lo, mid, up = tri_matrix()
@@ -175,7 +177,8 @@ def run_bm_voice(signal: np.ndarray, layout: VoiceLayout, ohc=0.5, max_m1_sp_err
                           sc.Ts, sc.alpha_l, sc.alpha_s, sc.Gme, sc.a0, sc.a1, sc.a2, sc.sigma_ow, sc.eta1, sc.eta2,
                           layout.time_blocks, bufP1, layout.overlap_nodes,
                           int(round(layout.overlap_nodes / layout.Fs * 1e6)), 1, sc.dt_max, sc.dt_min,
-                          decouple_filter, C.c_float(max_m1_sp_error), C.c_float(max_tolerance), relative_error, 1,
+                          decouple_filter, C.c_float(max_m1_sp_error), C.c_float(max_tolerance), relative_error,
+                          boundary_conditions,
                           C.c_float(m1_fix), C.c_float(tol_fix), C.c_float(spl_ref), 0, 0, 1 if timed else 0,
                           j_fast, j_slow, max_passes, 17, 2)
     check(rc, "BMOHCNewKernel")

@@ -182,7 +182,10 @@ void *getCudaLambda(void);
 void *getCudaBuffer(void);
 void *getCudaMeanRate(void);
 /* batched IHC/lambda over device pointers and an interval table (no
- * reference counterpart); d_scratch holds 2 x rows x 256 floats */
+ * reference counterpart); d_scratch holds 2 x rows x 256 floats.  With an IIR
+ * AC filter (d_ac_a) h_iir_ivs lists the runs the recursion restarts on:
+ * rows [row0, row0 + len), period len (CudaIIRFilterDecoupled,
+ * src/cochlea.cu:2885-2908) */
 int cm_ihc_lambda_run(const float *d_bm, int64_t rows, const cm_ihc_interval *h_ivs, int n_ivs,
                       const cm_ihc_interval *h_iir_ivs, int n_iir_ivs, const double *d_ac_b, const double *d_ac_a,
                       int n_ac, const double *d_dc, int n_dc, const double *d_damage, const float *d_A,
@@ -190,7 +193,12 @@ int cm_ihc_lambda_run(const float *d_bm, int64_t rows, const cm_ihc_interval *h_
                       float *d_scratch, float *d_lambda, float *d_stages);
 
 /* ---- JND --------------------------------------------------------------- */
-/* replaces CudaCalculateJND, include/cochlea_common.h:292-328 (src/cochlea.cu:3946-4065) */
+/* replaces CudaCalculateJND, include/cochlea_common.h:292-328 (src/cochlea.cu:3946-4065);
+ * backup_stage 1..6 (AvgMeanRate, dMeanRate, dLambda, preFisherAI,
+ * preFisherAITimeReduced, CRLB_RA; src/cochlea.cu:3860-3918) writes that
+ * stage to getCudaBuffer(): [3][JNDIntervals] for 1, [3][JNDIntervals][256]
+ * for 2, 5 and 6, and the interval bodies of the [3][rows][256] lambda layout
+ * for 3 and 4 */
 int CudaCalculateJND(int calculate_ai, int calculate_rms, int mean_size, int fisher_size, double SPLRefVal, double Fs,
                      double scaleBMVelocityForLambdaCalculation, const double *nIHC,
                      const int *JND_Calculated_Intervals, int numOFJNDCalculated, const int *JND_Refrence_Intervals,

@@ -786,7 +786,8 @@ static float halving_tree(float *a) {
 int orc_jnd(const float *lambda, int64_t rows, int n_intervals, int interval_nodes, int head_nodes,
             int length_nodes, const float *dA, const int32_t *ref_index, const double *nIHC, const float *W3,
             double Fs, double delta_alpha_factor, int calc_ai, int calc_rms, float *mean_rate, float *F_RA,
-            float *FisherAISum) {
+            float *FisherAISum, int backup_stage, float *buffer) {
+  const int stage = buffer ? backup_stage : 0;
   if (!lambda || n_intervals <= 0) return -1;
   const int len = length_nodes;
   std::vector<float> MR((size_t)ORC_LAMBDA_COUNT * n_intervals * NS);
@@ -806,10 +807,17 @@ int orc_jnd(const float *lambda, int64_t rows, int n_intervals, int interval_nod
     for (int iv = 0; iv < n_intervals; ++iv) {
       int ref = ref_index[iv];
       float dAv = dA[iv];
+      const size_t mi = ((size_t)li * n_intervals + iv) * NS;
+      if (stage == 1 && calc_rms) {          // AvgMeanRate (:3815-3831)
+        float tmp[NS];
+        std::memcpy(tmp, &MR[mi], sizeof(tmp));
+        buffer[(size_t)li * n_intervals + iv] = halving_tree(tmp) / (float)NS;
+      }
       for (int s = 0; s < NS; ++s) {
-        float mr = MR[((size_t)li * n_intervals + iv) * NS + s];
+        float mr = MR[mi + s];
         float mref = MR[((size_t)li * n_intervals + ref) * NS + s];
         float dMR = (mr - mref) / dAv;
+        if (stage == 2) buffer[mi + s] = dMR;   // dMeanRate (:3858-3861)
         if (calc_ai) {
           int64_t c = ((int64_t)li * rows + (int64_t)iv * interval_nodes + head_nodes) * NS + s;
           int64_t r = ((int64_t)li * rows + (int64_t)ref * interval_nodes + head_nodes) * NS + s;
@@ -819,13 +827,17 @@ int orc_jnd(const float *lambda, int64_t rows, int n_intervals, int interval_nod
             float cal = lambda[c + (int64_t)t * NS];
             float dl = dAv > 0 ? (cal - refl) / dAv : 0.0f;
             float pre = (float)((double)(dl * dl) / (Fs * (double)refl));
+            if (stage == 3) buffer[c + (int64_t)t * NS] = dl;    // dLambda (:3870-3872)
+            if (stage == 4) buffer[c + (int64_t)t * NS] = pre;   // preFisherAI (:3883-3885)
             accv += pre;
           }
           float rs = 1.0f / sqrtf(accv);
           acc_ai[s] = (float)((double)(T / Tlength) * nIHC[s] / (double)(rs * rs));
+          if (stage == 5) buffer[mi + s] = acc_ai[s];           // preFisherAITimeReduced (:3892-3894)
         }
         if (calc_rms) {
           float crlb = 1.0f / sqrtf(((T / mref) * dMR) * dMR);
+          if (stage == 6) buffer[mi + s] = crlb;                // CRLB_RA (:3909-3911)
           acc_rms[s] = (float)(nIHC[s] / (double)(crlb * crlb));
         }
       }

@@ -134,11 +134,14 @@ int orc_ihc_lambda(const float *bm, int64_t rows, int interval_len, int final_re
                    float *lambda, float *stages, int n_threads);
 
 /* JND Fisher reductions (src/cochlea.cu:3741-3944) and final dB
- * (src/AudioGramCreator.cpp:454-508). */
+ * (src/AudioGramCreator.cpp:454-508).  backup_stage 1..6 with a buffer writes
+ * that stage of GlobalCalculateJND (src/cochlea.cu:3815-3911): 1 AvgMeanRate
+ * [3][n]; 2 dMeanRate, 5 preFisherAITimeReduced, 6 CRLB_RA [3][n][256];
+ * 3 dLambda, 4 preFisherAI at the body nodes of [3][rows][256]. */
 int orc_jnd(const float *lambda, int64_t rows, int n_intervals, int interval_nodes, int head_nodes,
             int length_nodes, const float *dA, const int32_t *ref_index, const double *nIHC,
             const float *W3, double Fs, double delta_alpha_factor, int calc_ai, int calc_rms,
-            float *mean_rate, float *F_RA, float *FisherAISum);
+            float *mean_rate, float *F_RA, float *FisherAISum, int backup_stage, float *buffer);
 
 /* Profile input synthesis (src/cochlea.cu:4070-4103). */
 int orc_generate_input(int n_profiles, const double *dA, const double *Wn, const double *freq,

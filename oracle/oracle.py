@@ -217,19 +217,25 @@ def ihc_lambda(bm: np.ndarray, interval_len: int, frdp: int, ac_b: np.ndarray, d
 
 def jnd(lam: np.ndarray, n_intervals: int, interval_nodes: int, head_nodes: int, length_nodes: int,
         dA: np.ndarray, ref_index: np.ndarray, nIHC: np.ndarray, W: np.ndarray, Fs: float,
-        delta_alpha_factor: float, calc_ai: bool = True, calc_rms: bool = True):
+        delta_alpha_factor: float, calc_ai: bool = True, calc_rms: bool = True, backup_stage: int = 0):
+    """Mean rates, F_RA and FisherAISum; with backup_stage 1..6 also that
+    stage's buffer (NaN where the stage writes nothing)."""
     lam = np.ascontiguousarray(lam, np.float32)
     rows = lam.shape[1]
     mr = np.zeros((LAMBDA_COUNT, n_intervals, SECTIONS), np.float32)
     fra = np.zeros((LAMBDA_COUNT, n_intervals), np.float32)
     fai = np.zeros((LAMBDA_COUNT, n_intervals), np.float32)
+    shape = {1: (LAMBDA_COUNT, n_intervals), 3: lam.shape, 4: lam.shape}.get(
+        backup_stage, (LAMBDA_COUNT, n_intervals, SECTIONS))
+    buf = np.full(shape, np.nan, np.float32) if backup_stage else None
     rc = lib().orc_jnd(_p(lam), C.c_int64(rows), C.c_int(n_intervals), C.c_int(interval_nodes),
                        C.c_int(head_nodes), C.c_int(length_nodes), _p(np.ascontiguousarray(dA, np.float32)),
                        _p(np.ascontiguousarray(ref_index, np.int32)), _p(np.ascontiguousarray(nIHC, np.float64)),
                        _p(np.ascontiguousarray(W, np.float32)), C.c_double(Fs), C.c_double(delta_alpha_factor),
-                       C.c_int(int(calc_ai)), C.c_int(int(calc_rms)), _p(mr), _p(fra), _p(fai))
+                       C.c_int(int(calc_ai)), C.c_int(int(calc_rms)), _p(mr), _p(fra), _p(fai),
+                       C.c_int(backup_stage), _p(buf))
     assert rc == 0
-    return mr, fra, fai
+    return (mr, fra, fai, buf) if backup_stage else (mr, fra, fai)
 
 
 def generate_input(dA, Wn, freq, WN, wn_dc, wn_norm, sig_norm, interval_len, head, Fs):

@@ -243,12 +243,80 @@ def test_jnd_chain_end_to_end(oracle):
     check(L.GeneralKernel_Copy_Results_Bytes(ptr(mr_g), L.getCudaMeanRate(), mr_g.nbytes, 0), "mean rate copy")
     mr_g = mr_g.reshape(3, n_full, 256)
     _say(f"mean rate rel L2 {_rel(mr_g, mr):.3e} exact {np.mean(mr_g == mr):.6f}")
-    assert _rel(mr_g, mr) <= 1e-6
+    np.testing.assert_array_equal(mr_g, mr)
     sel = calc
     _say(f"JND AI dB gpu {np.round(ai[serial[sel]], 3)} oracle {np.round(ai_o[serial[sel]], 3)}")
     assert np.max(np.abs(ai[serial[sel]] - ai_o[serial[sel]])) <= 0.1
     assert np.max(np.abs(rate[serial[sel]] - rate_o[serial[sel]])) <= 0.1
     assert np.all(np.isfinite(ai[serial[sel]]))
+    # backup stages 1..6 of CudaCalculateJND (TEST_File_Target) land in getCudaBuffer()
+    W3 = np.array(stages.W_DEFAULT, np.float32)
+    body = np.concatenate([np.arange(i * c["L"] + c["head"], i * c["L"] + c["head"] + c["length"])
+                           for i in range(n_full)])
+    for k in range(1, 7):
+        check(L.CudaCalculateJND(1, 1, 0, 3 * n_full, 2e-5, FS, 1.0, ptr(nIHC), ptr(calc), len(calc), ptr(refs),
+                                 len(refs), 0, n_full, n_full, c["head"], c["ov"], c["L"], c["length"], ptr(serial),
+                                 ptr(to_ref), ptr(F_RA), ptr(FAI), rows * 256, rows * 256, 0.2 * FS, 1, 0, 0, 0, k),
+              f"CudaCalculateJND stage {k}")
+        exp = oracle.jnd(lo, n_full, c["L"], c["head"], c["length"], dA, ref_idx, nIHC, W3, FS, 0.2 * FS,
+                         backup_stage=k)[3]
+        got = np.zeros(exp.size, np.float32)
+        check(L.GeneralKernel_Copy_Results_Bytes(ptr(got), L.getCudaBuffer(), got.nbytes, 0), "stage copy")
+        got = got.reshape(exp.shape)
+        if k in (3, 4):
+            got, exp = got[:, body], exp[:, body]
+        fin = np.isfinite(exp)
+        _say(f"JND stage {k}: {exp.size} values, exact {np.mean(got[fin] == exp[fin]):.6f}")
+        np.testing.assert_array_equal(got, exp)
+
+
+def test_cuda_calculate_jnd_refuses_bad_layouts():
+    """CudaCalculateJND checks its layout against the buffers before any launch:
+    lambda planes beyond what RunIHCKernel sized, interval bodies past their
+    interval, columns past JNDIntervalsFull, serial positions outside the
+    calculated-interval table and dA reads past the BM input buffer are refused
+    with -3 and a message."""
+    from cochlearmodel_amd import _lib, solver, stages
+    from cochlearmodel_amd._lib import check, ptr
+    _say("case: CudaCalculateJND argument refusals (100-row lambda buffer)")
+    L = _lib.lib()
+    # fresh holders: the grow-only buffers of earlier tests must not hide a refusal
+    check(L.IHCKernel_Free(), "IHCKernel_Free")
+    check(L.BMOHCKernel_Free(), "BMOHCKernel_Free")
+    sec = solver.section_params()
+    g = np.full(256, 0.5, np.float32)
+    keep = [np.ascontiguousarray(x, np.float32) for x in (g, sec[0], sec[1], g, g, sec[2], sec[3], sec[4], sec[5],
+                                                          sec[6], sec[7])]
+    check(L.BMOHCKernel_Init(*[ptr(k) for k in keep], 1.0, 0.0, 400, 400 * 256, 0, 1, 0, 0), "init")
+    b, _ = stages.load_ac_filter(FS)
+    dc = stages.dc_filter(FS)
+    dmg = stages.ihc_damage(8.0)
+    nc = np.array([60, 3, 0.1, 0, 0, 0, 0.61, 0.23, 0.16], np.float32)
+    check(L.IHCNewKernel(ptr(dmg), ptr(nc), ptr(dc), len(dc), ptr(b), None, 0, len(b), 256, 2, 2e-5, None, 0, 100, 100,
+                         100, 50, 0, 0, 500.0, 1.0, 100.0, 1, 1, 0, 0, 0, 0, 1.0, 1, 1, 0, 1, 0), "IHCNewKernel")
+    nIHC = np.full(256, 30000 / 256, np.float64)
+    F = np.zeros(3 * 4, np.float32)
+    refs = np.array([1], np.int32)
+    calc = np.array([0], np.int32)
+
+    def call(matrix_rows=100, n_iv=2, full=2, handled=0, head=10, length=30, serial=(0, 0), calcdA=0, overlap=0):
+        ser = np.array(serial, np.int32)
+        to_ref = np.array([1], np.int32)
+        return L.CudaCalculateJND(1, 1, 0, 3 * full, 2e-5, FS, 1.0, ptr(nIHC), ptr(calc), 1, ptr(refs), 1, handled,
+                                  full, n_iv, head, overlap, 50, length, ptr(ser), ptr(to_ref), ptr(F), ptr(F),
+                                  matrix_rows * 256, matrix_rows * 256, 0.2 * FS, 1, 0, calcdA, 0, 0)
+
+    def msg():
+        return L.cm_last_error().decode()
+
+    assert call(matrix_rows=200) == -3 and "lambda buffer" in msg()
+    assert call(head=10, length=45) == -3 and "outside an interval" in msg()
+    assert call(head=-1) == -3 and "outside an interval" in msg()
+    assert call(handled=1) == -3 and "JNDIntervalsFull" in msg()
+    assert call(serial=(5, 0)) == -3 and "serial position" in msg()
+    assert call(calcdA=1, overlap=1000) == -3 and "dA reads past" in msg()
+    check(L.IHCKernel_Free(), "IHCKernel_Free")
+    check(L.BMOHCKernel_Free(), "BMOHCKernel_Free")
 
 
 @pytest.mark.parametrize("tau,ws", [(1e-4, 0), (1e-6, 0), (1e-6, 1)])

@@ -289,6 +289,86 @@ def test_complex_jnd_run_files(tmp_path):
         np.testing.assert_array_equal(vel3[p_i][:-1], vel_all[j][:-1], err_msg=f"profile {p_i}")
 
 
+def test_stage_names_are_checked_before_any_device_call(tmp_path):
+    """TEST_File_Target / Type_TEST_Output: an unknown stage name, a lambda plane
+    past the third and a JND stage in a run without Calculate_JND are refused
+    while the parameters are read."""
+    jnd = ("Sim_Type = 4\nCalculate_JND = 1\ntestedFrequencies = 1000\ntestedPowerLevels = 20 40\n"
+           "JND_File_Name = j.bin\nTEST_File_Target = t.bin\n")
+    voice = "Input_Signal = 0 0 0 0\nOutput_Result = o.bin\nTEST_File_Target = t.bin\n"
+    for text, err in ((jnd + "Type_TEST_Output = dLambdaX\n", "unknown Type_TEST_Output"),
+                      (jnd + "Type_TEST_Output = JND_Lambda3\n", "must be smaller than 3"),
+                      (voice + "Type_TEST_Output = CRLB_RA\n", "needs Calculate_JND"),
+                      (voice, "needs Calculate_JND")):       # the default MeanRate
+        p = tmp_path / "s.par"
+        p.write_text(text)
+        r = subprocess.run([EXE, str(p)], capture_output=True, text=True, timeout=60)
+        assert r.returncode == 1 and err in r.stderr, (text, r.stderr)
+
+
+@pytest.mark.gpu
+def test_jnd_stage_dumps(tmp_path, oracle):
+    """TEST_File_Target dumps of a Sim_Type 4 run through cochlea_run: the JND
+    stages (AvgMeanRate, dMeanRate, dLambda of one plane, preFisherAI,
+    preFisherAITimeReduced, CRLB_RA, MeanRate) against the oracle's stages on
+    the same lambda, a lambda plane and the IHC stage against the oracle's IHC
+    chain on the same BM velocity, bit for bit."""
+    from cochlearmodel_amd import solver, stages
+    from cochlearmodel_amd.jnd_sweep import JndSweep, N_IHC
+    print("case: cochlea_run Sim_Type 4 stage dumps, 2 freq x 3 levels x 2 noises, 40 ms intervals", flush=True)
+    sw = JndSweep((1000.0, 3000.0), (20.0, 40.0, 60.0), (1111.0, 40.0), duration=0.04)
+    sw.step(solver.default_scalars(FS), timed_bm=False)
+    lam = sw.d_lambda.cpu().numpy()
+    speeds = sw.d_speeds[:sw.rows].cpu().numpy()
+    wn = tmp_path / "noise.f64"
+    sw.WN.astype(np.float64).tofile(wn)
+    common = ("Sim_Type = 4\nCalculate_JND = 1\nFs = 44100\ntestedFrequencies = 1000 3000\n"
+              "testedPowerLevels = 20 40 60\ntestedNoises = 1111 40\nJND_Interval_Duration = 0.04\n"
+              f"Input_Noise_File = {wn}\nJND_File_Name = {tmp_path / 'jnd.bin'}\n")
+    nIHC = np.full(256, N_IHC, np.float64)
+    W3 = np.array(stages.W_DEFAULT, np.float32)
+    dA = sw.profiles["dA"].astype(np.float32)
+
+    def oracle_stage(k):
+        return oracle.jnd(lam, sw.n_prof, sw.interval, sw.head, sw.length, dA, sw.ref_index, nIHC, W3, FS,
+                          0.2 * FS, backup_stage=k)[3]
+
+    body = np.concatenate([np.arange(i * sw.interval + sw.head, i * sw.interval + sw.head + sw.length)
+                           for i in range(sw.n_prof)])
+    b, _ = stages.load_ac_filter(FS)
+    _, ihc_st = oracle.ihc_lambda(speeds, sw.interval, sw.block * sw.TB, b, stages.dc_filter(FS),
+                                  stages.ihc_damage(8.0), stages.aihc_table(),
+                                  np.array(stages.SPONT_DEFAULT, np.float32), stages=True, threads=8)
+    mr = oracle.jnd(lam, sw.n_prof, sw.interval, sw.head, sw.length, dA, sw.ref_index, nIHC, W3, FS, 0.2 * FS)[0]
+    cases = {
+        "AvgMeanRate": lambda: oracle_stage(1),
+        "dMeanRate": lambda: oracle_stage(2),
+        "dLambda1": lambda: oracle_stage(3)[1][body],
+        "preFisherAI": lambda: oracle_stage(4)[:, body],
+        "preFisherAITimeReduced": lambda: oracle_stage(5),
+        "CRLB_RA": lambda: oracle_stage(6),
+        "MeanRate": lambda: mr,
+        "JND_Lambda2": lambda: lam[2],
+        "IHC": lambda: ihc_st[4],
+    }
+    for name, expect in cases.items():
+        tgt = tmp_path / f"{name}.bin"
+        (tmp_path / "s.par").write_text(common + f"TEST_File_Target = {tgt}\nType_TEST_Output = {name}\n")
+        r = subprocess.run([EXE, str(tmp_path / "s.par")], capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, (name, r.stdout + r.stderr)
+        exp = expect()
+        got = np.fromfile(tgt, np.float32)
+        if name == "dLambda1":
+            got = got.reshape(sw.rows, 256)[body]
+        elif name == "preFisherAI":
+            got = got.reshape(3, sw.rows, 256)[:, body]
+        else:
+            got = got.reshape(exp.shape)
+        print(f"{name}: {exp.size} values, exact {np.mean((got == exp) | (np.isnan(got) & np.isnan(exp))):.6f}",
+              flush=True)
+        np.testing.assert_array_equal(got, exp, err_msg=name)
+
+
 def test_profile_layout_without_jnd(tmp_path):
     """Calculate_JND = 0: no reference intervals, reference indexes -1."""
     p = tmp_path / "p.par"

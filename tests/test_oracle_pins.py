@@ -310,6 +310,44 @@ def test_jnd_analytic(oracle):
     assert abs(ai[0] - exp) < 1e-9
 
 
+def test_jnd_backup_stages_analytic(oracle):
+    """The six backup stages of GlobalCalculateJND (src/cochlea.cu:3815-3911)
+    in closed form for a constant reference rate and a constant lambda step d
+    over the interval body."""
+    n_int, L, head, ln = 2, 50, 5, 40
+    rows = n_int * L
+    lam = np.zeros((3, rows, 256), np.float32)
+    base = np.array([60.0, 3.0, 0.1], np.float32)
+    d = np.float32(0.5)
+    for j in range(3):
+        lam[j, 0:L] = base[j] + d
+        lam[j, L:2 * L] = base[j]
+    dA = np.array([2.0, 0.0], np.float32)
+    ref = np.array([1, 1], np.int32)
+    nIHC = np.full(256, 30000 / 256, np.float64)
+    W = np.array([0.61, 0.23, 0.16], np.float32)
+    Fs = 44100.0
+    T = 0.2 * Fs
+    Tf, Tl = T / Fs, ln / Fs
+    st = {k: oracle.jnd(lam, n_int, L, head, ln, dA, ref, nIHC, W, Fs, T, backup_stage=k)[3] for k in range(1, 7)}
+    body = slice(head, head + ln)
+    for j in range(3):
+        dl = d / dA[0]
+        np.testing.assert_allclose(st[1][j, 0], base[j] + d, rtol=1e-6)          # AvgMeanRate
+        np.testing.assert_allclose(st[1][j, 1], base[j], rtol=1e-6)
+        np.testing.assert_allclose(st[2][j, 0], dl, rtol=1e-5)                   # dMeanRate
+        assert np.all(np.isnan(st[2][j, 1]))                                     # reference: 0 / dA = 0 / 0
+        np.testing.assert_allclose(st[3][j, body], dl, rtol=1e-5)                # dLambda at the body nodes
+        assert np.all(np.isnan(st[3][j, :head])) and np.all(np.isnan(st[3][j, head + ln:L]))
+        assert np.all(st[3][j, L + head:L + head + ln] == 0)
+        np.testing.assert_allclose(st[4][j, body], dl ** 2 / (Fs * base[j]), rtol=1e-5)   # preFisherAI
+        np.testing.assert_allclose(st[5][j, 0], (Tf / Tl) * nIHC[0] * ln * dl ** 2 / (Fs * base[j]), rtol=1e-4)
+        np.testing.assert_allclose(st[6][j, 0], 1.0 / (dl * np.sqrt(Tf / base[j])), rtol=1e-5)   # CRLB_RA
+    # without calculate_rms the reference writes no AvgMeanRate
+    assert np.all(np.isnan(oracle.jnd(lam, n_int, L, head, ln, dA, ref, nIHC, W, Fs, T, calc_rms=False,
+                                      backup_stage=1)[3]))
+
+
 def test_input_generation_closed_form(oracle):
     rng = np.random.default_rng(5)
     L, head, Fs = 300, 20, 44100.0
