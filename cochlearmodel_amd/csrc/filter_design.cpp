@@ -12,7 +12,9 @@
 // minimum order (Oppenheim & Schafer eq. 7.104, SubModel::calcMinimumOrder),
 // band weights from the ripples, DC-gain normalisation (fixDCGain), the -Apass
 // corner correction (FIRFreqError, src/firpm/pm.cpp:742-793) and a redesign,
-// then unit FIR gain (HFunction::setFIRGain(0), src/HFunction.cpp:43-46).
+// the tap search (Fpass/Fstop) or transition-width search (Fc with Astop) that
+// brings the stop-band ripple within a factor 3 of the Astop target, then unit
+// FIR gain (HFunction::setFIRGain(0), src/HFunction.cpp:43-46).
 #include <algorithm>
 #include <cctype>
 #include <cmath>
@@ -469,21 +471,34 @@ int cm_design_ac_filter(double Fs, const char *function_filter, double *h, doubl
     cm::set_error("cm_design_ac_filter: filter type '%s' is not supported", type.c_str());
     return -3;
   }
-  if (kv.count("fc") || !kv.count("fpass") || !kv.count("fstop")) {
-    cm::set_error("cm_design_ac_filter: needs Fpass and Fstop (the Fc form is not supported)");
+  // equiripple branch of SubModel::analyzeACFilter (src/SubModel.cpp:167-360)
+  const bool has_fc = kv.count("fc") > 0;
+  if (has_fc && (kv.count("fpass") || kv.count("fstop"))) {   // src/params.cpp:1136-1140
+    cm::set_error("cm_design_ac_filter: Fc cannot be given together with Fpass/Fstop");
+    return -3;
+  }
+  if (!has_fc && (!kv.count("fpass") || !kv.count("fstop"))) {
+    cm::set_error("cm_design_ac_filter: equiripple needs either Fpass,Fstop or Fc");
     return -3;
   }
   const double nyq = (double)(float)(Fs / 2);
   const double Fpass = (float)num("fpass", 600.0), Fstop = (float)num("fstop", 1600.0);
-  double omega_c = Fpass / nyq;
-  const double pw = (Fstop - Fpass) / nyq;
-  const double max_fstop = Fpass + nyq * 0.3, min_fstop = Fpass + nyq * 0.01;
+  double omega_c, pw = 0.0, max_fstop = 0.0, min_fstop = 0.0;
+  if (has_fc) {
+    omega_c = (double)(float)num("fc", 0.0) / nyq;
+  } else {
+    omega_c = Fpass / nyq;
+    pw = (Fstop - Fpass) / nyq;
+    max_fstop = Fpass + nyq * 0.3;
+    min_fstop = Fpass + nyq * 0.01;
+  }
   int order = kv.count("order") && kv["order"] != "minimum" ? std::atoi(kv["order"].c_str()) : -1;
   const bool min_order = order == -1 || (kv.count("minorder") && std::atoi(kv["minorder"].c_str()) == 1);
   const bool deltas_found = kv.count("astop") || kv.count("apass") ||
                             (kv.count("mode") && kv["mode"].find("constrained") != std::string::npos);
-  // Apass is 1 dB for the ripple when only Astop is given, the corner
-  // correction then still aims at -3 dB (src/SubModel.cpp:210-213, 273)
+  // Apass is 1 dB for the ripple when only Astop is given; the first corner
+  // correction still aims at -3 dB, the ones inside the searches at -Apass
+  // (src/SubModel.cpp:210-213, 273, 317, 339)
   const double Apass = (float)num("apass", 1.0);
   const double corner_db = kv.count("apass") ? Apass : 3.0;
   double dp, ds;
@@ -500,33 +515,92 @@ int cm_design_ac_filter(double Fs, const char *function_filter, double *h, doubl
     dp = 1.0;
     ds = 10.0;
   }
-  if (min_order) order = (int)std::ceil((-10.0 * std::log10(dp * ds) - 13.0) / (2.324 * pw * kPi));
+  // Fc with Astop: the transition width is searched, from 0.15 between
+  // MIN_PARK_WIDTH 0.01 and MAX_PARK_WIDTH 0.3 (include/const.h:344-345)
+  const bool search_pw = has_fc && kv.count("astop");
+  if (search_pw) pw = 0.15;
+  if (min_order) {
+    if (!(pw > 0.0)) {
+      cm::set_error("cm_design_ac_filter: a minimum order needs a transition band (Fc needs Order or Astop)");
+      return -3;
+    }
+    order = (int)std::ceil((-10.0 * std::log10(dp * ds) - 13.0) / (2.324 * pw * kPi));
+  }
   double w_stop = 10.0;
   double w_pass = std::max(0.01, 10.0 * ds / dp);
   if (w_pass > 10.0 * ds / dp) w_stop *= w_pass / (10.0 * ds / dp);
   if (kv.count("wstop")) w_stop = (float)num("wstop", 100.0);
   if (kv.count("wpass")) w_pass = (float)num("wpass", 1.0);
-  if (Fstop > max_fstop || Fstop < min_fstop) {
+  if (!has_fc && (Fstop > max_fstop || Fstop < min_fstop)) {
     cm::set_error("cm_design_ac_filter: Fstop should be between %g and %g", min_fstop, max_fstop);
     return -3;
   }
-  const int taps = order + 1;   // firpm's degree n gives n + 1 coefficients
-  if (order < 2 || taps > max_taps) {
-    cm::set_error("cm_design_ac_filter: %d taps (room for %d)", taps, max_taps);
-    return -3;
-  }
-  std::vector<double> b(taps);
-  auto design = [&](double wc) {
-    if (cm_remez_lowpass(taps, 0.5 * wc, 0.5 * (wc + pw), w_pass, w_stop, 16, b.data(), nullptr, nullptr)) return -3;
+  // One design of `order` (firpm's degree n gives n + 1 coefficients) at cut-off
+  // wc and transition width; rip: the achieved stop-band ripple of the
+  // DC-normalised result (the exchange's equiripple deviation over the stop
+  // weight).  A zero width (Fc without Astop) starts the stop band one grid
+  // step above the pass band edge.
+  std::vector<double> b;
+  double rip = 0.0;
+  auto design = [&](int n, double wc, double width) {
+    const int taps = n + 1;
+    if (n < 2 || taps > max_taps) {
+      cm::set_error("cm_design_ac_filter: %d taps (room for %d)", taps, max_taps);
+      return -3;
+    }
+    b.assign(taps, 0.0);
+    const int nfcns = taps % 2 == 0 ? taps / 2 : taps / 2 + 1;
+    const double fp = 0.5 * wc;
+    const double fs = width > 0.0 ? 0.5 * (wc + width) : fp + 0.5 / (16.0 * nfcns);
+    double dev = 0.0;
+    if (cm_remez_lowpass(taps, fp, fs, w_pass, w_stop, 16, b.data(), &dev, nullptr)) return -3;
     const double dc = fir_gain(b, 0.0);   // fixDCGain(pmo, 1)
     for (double &x : b) x /= dc;
+    rip = dev / w_stop / dc;
     return 0;
   };
-  if (design(omega_c)) return -3;
+  if (design(order, omega_c, pw)) return -3;
+  const double omega_c0 = omega_c;
   if (deltas_found) {
     omega_c = corner_correction(b, omega_c, corner_db);
-    if (design(omega_c)) return -3;
+    if (design(order, omega_c, pw)) return -3;
   }
+  // each search step: the design at the locked cut-off, its corner correction
+  // and the redesign (src/SubModel.cpp:314-320, 336-342)
+  auto redesign = [&]() {
+    if (design(order, omega_c0, pw)) return -3;
+    omega_c = corner_correction(b, omega_c0, Apass);
+    return design(order, omega_c, pw);
+  };
+  // The searches stop once the stop-band ripple is within a factor 3 of the
+  // Astop target.  The reference tests firpm's convergence measure Q there
+  // (pmo.Q, src/firpm/pm.cpp:602-603), which is not a ripple and depends on
+  // the exchange's iteration history; the achieved ripple is the quantity the
+  // comparisons and the 10 log10 tap rule are written for.
+  if (search_pw) {                               // src/SubModel.cpp:280-324
+    double pw_high = 0.3, pw_low = 0.01;
+    for (int tests = 0; (rip > 3 * ds || 3 * rip < ds) && tests < 10; ++tests) {
+      if (min_order && (rip > 10 * ds || 10 * rip < ds)) {
+        order += (int)std::ceil(10 * std::log10(rip / ds));
+      } else if (rip > 3 * ds) {                 // too little attenuation: widen the transition
+        pw_low = pw;
+        pw = (pw + pw_high) / 2;
+      } else {                                   // more than needed: narrow it
+        pw_high = pw;
+        pw = (pw + pw_low) / 2;
+      }
+      if (redesign()) return -3;
+    }
+  } else if (!has_fc && !kv.count("wstop")) {    // src/SubModel.cpp:325-346
+    double fix = 10, prev = rip;
+    for (int tests = 0; (rip > 3 * ds || 3 * rip < ds) && tests < 5; ++tests) {
+      if ((prev > ds && rip < ds) || (prev < ds && rip > ds)) fix *= 0.5;
+      prev = rip;
+      order += (int)std::ceil(fix * std::log10(rip / ds));
+      if (redesign()) return -3;
+    }
+  }
+  const int taps = (int)b.size();
   double sum = 0.0;   // setFIRGain(0): unit gain at DC
   for (double x : b) sum += x;
   for (int i = 0; i < taps; ++i) h[i] = b[i] / sum;

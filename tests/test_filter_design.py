@@ -2,6 +2,8 @@
 against scipy.signal.remez on the same grid, its equiripple alternation, and
 the reference's equiripple procedure around it (minimum order, unit DC gain,
 error cases of SubModel::analyzeACFilter)."""
+import math
+
 import numpy as np
 import pytest
 
@@ -82,12 +84,62 @@ def test_ac_filter_design_procedure():
     np.testing.assert_array_equal(h, h2)
 
 
-@pytest.mark.parametrize("spec", ["type=elliptic,fpass=600,fstop=1600", "type=equiripple,fc=800,astop=60",
-                                  "type=equiripple,fpass=600,fstop=9000,astop=60", "type=equiripple,fpass=600"])
+@pytest.mark.parametrize("spec", ["type=elliptic,fpass=600,fstop=1600", "type=equiripple,fc=800,fpass=600,astop=60",
+                                  "type=equiripple,fpass=600,fstop=9000,astop=60", "type=equiripple,fpass=600",
+                                  "type=equiripple,fc=800", "type=equiripple,fc=800,wstop=100"])
 def test_ac_filter_design_refuses(spec):
     from cochlearmodel_amd import _lib
     rc, _ = _design(44100.0, spec)
     assert rc == -3 and _lib.lib().cm_last_error()
+
+
+@pytest.mark.parametrize("astop", [40, 50, 60, 70, 80])
+def test_equiripple_order_search(astop):
+    """The tap search of src/SubModel.cpp:325-346 (Fpass/Fstop with Astop): the
+    order moves by ceil(fix 10 log10(ripple / delta_stop)) until the stop-band
+    ripple is within a factor 3 of delta_stop = 10^(-Astop/20).  Measured deep
+    in the stop band (above 5 kHz), where the equiripple error is the design's
+    ripple.  At 40 dB the minimum order of O&S 7.104 misses the target, and the
+    search adds taps."""
+    rc, h = _design(44100.0, f"type=equiripple,fpass=600,fstop=1600,astop={astop}")
+    assert rc == 0
+    ratio = _gain(h, np.linspace(5000, 22050, 4000), 44100.0).max() / 10 ** (-astop / 20)
+    assert 1 / 3 <= ratio <= 3, ratio
+    np.testing.assert_array_equal(h, h[::-1])
+    assert abs(h.sum() - 1.0) < 1e-12
+    if astop == 40:
+        dp, ds = 10 ** (-1 / 20), 1e-2       # Apass defaults to 1 dB when only Astop is given
+        min_taps = math.ceil((-10 * math.log10(dp * ds) - 13) / (2.324 * math.pi * 1000 / 22050)) + 1
+        assert min_taps == 24 and len(h) > min_taps
+
+
+@pytest.mark.parametrize("astop", [40, 60, 80])
+def test_equiripple_fc_with_astop(astop):
+    """Fc with Astop (src/SubModel.cpp:181-183, 219-220, 280-324): the transition
+    width is searched from 0.15 of Nyquist by bisection towards 0.3 or 0.01
+    until the stop-band ripple is within a factor 3 of the target; the
+    minimum order follows from the starting width.  The corner correction aims
+    the -Apass (1 dB) point at Fc."""
+    rc, h = _design(44100.0, f"type=equiripple,fc=800,astop={astop}")
+    assert rc == 0
+    # the stop band starts at most 0.3 x 22050 above Fc
+    ratio = _gain(h, np.linspace(800 + 0.3 * 22050, 22050, 4000), 44100.0).max() / 10 ** (-astop / 20)
+    assert 1 / 3 <= ratio <= 3, ratio
+    np.testing.assert_array_equal(h, h[::-1])
+    assert abs(h.sum() - 1.0) < 1e-12
+    g = _gain(h, np.array([0.0, 200.0]), 44100.0)
+    assert abs(g[0] - 1) < 1e-12 and g[1] > 10 ** (-1.5 / 20)
+
+
+def test_equiripple_fc_with_order():
+    """Fc with an order and no Astop: zero transition width (the reference's
+    bands [0, wc] and [wc, 1] touch), taken here as a stop band starting one
+    grid step above wc; the result is a linear-phase design of order + 1 taps
+    with unit DC gain, whatever its (poor) selectivity."""
+    rc, h = _design(44100.0, "type=equiripple,fc=800,order=60")
+    assert rc == 0 and len(h) == 61
+    np.testing.assert_array_equal(h, h[::-1])
+    assert abs(h.sum() - 1.0) < 1e-12
 
 
 def _gain(h, f_hz, fs):
