@@ -3,17 +3,18 @@ GELU MLP trained and run on the GPU with the library's own BF16-MFMA GEMM and
 elementwise kernels (FP32 master weights, FP32 accumulation).  torch tensors
 only hold device memory.  No reference counterpart.
 
-Features of one solver pass (the pressure solve after the Euler predictor):
-  x = [d * c_d / a, v * c_v / a, p_prev / a, s * c_s / a]   (256+256+256+1)
-with a = RMS(p_prev) + tiny, and the target is the pressure change
-(p* - p_prev) / (a * c_out), where p* solves A p = Y exactly.  The constants
-c_* make each block O(1) on the training data; a single per-sample scale keeps
-the near-linear (low-level) regime scale-equivariant.
+Inputs of one predictor-pass pressure solve (built on the GPU by the solver
+kernel, see warmstart.py): the 256-section displacement d and velocity v of the
+Euler predictor, each divided by its RMS over the sections; the residual
+r_lin = Y - A p_lin of the extrapolated pressure p_lin = p + (dt / dt_prev)(p - p_prev),
+divided by its RMS; and the input sample s divided by RMS(p)  (256+256+256+1).
+The last pressure enters through p_lin and r_lin.  The output y is a
+correction: the guess is p_lin + RMS(r_lin) c_out S(y), S = [1,2,1]/4, and the
+training target is (A^-1 Y - p_lin) / (RMS(r_lin) c_out).
 """
 from __future__ import annotations
 
 import ctypes as C
-import json
 import math
 
 import numpy as np
@@ -49,7 +50,6 @@ class WarmStartMLP:
         self.mb = [torch.zeros_like(b) for b in self.b]
         self.vb = [torch.zeros_like(b) for b in self.b]
         self.t = 0
-        self.norm = {"c_d": 1.0, "c_v": 1.0, "c_s": 1.0, "c_out": 1.0}
         self._cap = 0
 
     # ---------------------------------------------------------------- utils
@@ -153,34 +153,3 @@ class WarmStartMLP:
             if log:
                 log(f"epoch {ep}: mse {hist[-1]:.4e}")
         return hist
-
-    def save(self, path):
-        np.savez(path, *self.params_numpy(), norm=json.dumps(self.norm))
-
-    @classmethod
-    def load(cls, path, device="cuda"):
-        z = np.load(path)
-        params = [z[f"arr_{i}"] for i in range(6)]
-        m = cls(device=device, params=params)
-        m.norm = json.loads(str(z["norm"]))
-        return m
-
-
-# ---------------------------------------------------------------- features
-def features(d, v, p_prev, s, norm):
-    """Per-sample features and scale (numpy or torch, rows = samples)."""
-    xp = np if isinstance(d, np.ndarray) else __import__("torch")
-    a = xp.sqrt((p_prev * p_prev).mean(1, keepdims=True) if xp is np else (p_prev * p_prev).mean(1, keepdim=True))
-    a = a + 1e-30
-    s = s.reshape(-1, 1)
-    parts = [d * norm["c_d"] / a, v * norm["c_v"] / a, p_prev / a, s * norm["c_s"] / a]
-    X = np.concatenate(parts, 1) if xp is np else xp.cat(parts, 1)
-    return X, a
-
-
-def fit_normalisation(d, v, p_prev, s, p_star):
-    """Constants that make each feature block (and the target) O(1)."""
-    rms = lambda x: float(np.sqrt(np.mean(np.asarray(x, np.float64) ** 2)) + 1e-300)
-    a = np.sqrt((p_prev.astype(np.float64) ** 2).mean(1, keepdims=True)) + 1e-30
-    return {"c_d": rms(p_prev / a) / rms(d / a), "c_v": rms(p_prev / a) / rms(v / a),
-            "c_s": rms(p_prev / a) / rms(s.reshape(-1, 1) / a), "c_out": rms((p_star - p_prev) / a)}

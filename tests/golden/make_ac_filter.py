@@ -7,7 +7,9 @@ default band spec (src/params.cpp:61-68: Fpass 600 Hz, Fstop 1600 Hz, Apass
 scipy.signal.remez; the order is the Herrmann/Kaiser minimum-order estimate,
 the band weights follow the reference's rule (stop weight 10, pass weight
 10*delta_stop/delta_pass, src/SubModel.cpp:224-232).
-Run: python cochlearmodel_amd/data/make_ac_filter.py
+The fixtures it writes live in cochlearmodel_amd/data/, where the library and
+the tests read them.
+Run: python tests/golden/make_ac_filter.py
 """
 import json
 import math
@@ -16,7 +18,8 @@ import os
 import numpy as np
 from scipy import signal
 
-HERE = os.path.dirname(os.path.abspath(__file__))
+HERE = os.path.join(os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__)))),
+                    "cochlearmodel_amd", "data")
 
 
 def design(fs=44100.0, fpass=600.0, fstop=1600.0, apass=3.0, astop=60.0):
