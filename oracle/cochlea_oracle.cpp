@@ -251,7 +251,10 @@ template <typename T> struct Outs {
 };
 
 // One time block integrated by kernel #17 (include/cochlea.cuh:13573-14245).
-template <typename T>
+// C is the type of the sample clock (local time t, time_step_out, Ts and the
+// interpolation weight): the method keeps it in float; the FP64 rebuild takes
+// it as double or, to isolate the clock's rounding (Q9), as float.
+template <typename T, typename C = T>
 void bm_item(const orc_bm_scalars &sc, const float *sec, const float *gamma_set, const orc_bm_item &it,
              int item_index, const float *input, int64_t input_len, const Outs<T> &o, Trace *tr = nullptr) {
   // per-section parameters (:13700-13736); promoted from the packed FP32 values
@@ -270,10 +273,11 @@ void bm_item(const orc_bm_scalars &sc, const float *sec, const float *gamma_set,
     rmass[i] = (T)1 / mass;                 // reciprocal_mass_ind = 1.0f / mass[i]
     low[i] = i < NS - 2 ? (T)1 : (T)0;      // low_multiplier, SECTIONSM2
   }
-  const T w_ohc = sc.w_ohc, dx = sc.delta_x, alpha_r = sc.alpha_r, Ts = sc.Ts;
+  const T w_ohc = sc.w_ohc, dx = sc.delta_x, alpha_r = sc.alpha_r;
   const T al = sc.alpha_l, as = sc.alpha_s, Gme = sc.Gme, a0 = sc.a0, a1 = sc.a1, a2 = sc.a2;
   const T sigma_ow = sc.sigma_ow, eta1 = sc.eta1, eta2 = sc.eta2;
-  const T dt_min = sc.dt_min, dt_max = sc.dt_max, tso = sc.time_step_out;
+  const T dt_min = sc.dt_min, dt_max = sc.dt_max;
+  const C tso = (C)sc.time_step_out, Ts_c = (C)sc.Ts;
   const T mtt = it.tol, mmspt = it.m1tol;
   const T half = (T)0.5f, one = (T)1;
   const T al_over_as = al / as;             // __fdividef(_ohc_alpha_l, _ohc_alpha_s)
@@ -292,7 +296,7 @@ void bm_item(const orc_bm_scalars &sc, const float *sec, const float *gamma_set,
 
   T dt = (T)sc.time_step;
   T dt_prev = dt;   // dt of the previous step (warm_start 1)
-  T t = 0;
+  C t = 0;
   int in_s = it.start;
   auto in_at = [&](int64_t idx) -> T { return (idx >= 0 && idx < input_len) ? (T)input[idx] : (T)0; };
   T sx0 = in_at(in_s), sx1 = in_at(in_s + 1);
@@ -354,7 +358,7 @@ void bm_item(const orc_bm_scalars &sc, const float *sec, const float *gamma_set,
         G[i] = -g - Ptm[i];
       }
       // right-hand side and boundaries (:13925-13954)
-      T delta = t / Ts;
+      T delta = (T)(t / Ts_c);
       T s = sx1 * delta + sx0 * (one - delta);
       for (int i = 1; i < NS - 1; ++i) Y[i] = (dxdx * G[i]) * Q[i];
       Y[0] = dxa0 * ((((-a2) * ow_nv) - a1 * ow_nd) + Gme * s);
@@ -487,7 +491,7 @@ void bm_item(const orc_bm_scalars &sc, const float *sec, const float *gamma_set,
       ppsid[i] = npsid[i]; pxis[i] = nxis[i]; pxi[i] = nxi[i];
     }
     ow_pd = ow_nd; ow_pv = ow_nv; ow_pa = ow_na;
-    t += dt_commit;   // the new dt, also after a halving pass (Q2)
+    t += (C)dt_commit;   // the new dt, also after a halving pass (Q2)
   }
   if (o.failed_blocks) o.failed_blocks[item_index] = fb;
   if (o.conv_time_blocks) o.conv_time_blocks[item_index] = ctb;
@@ -498,7 +502,7 @@ void bm_item(const orc_bm_scalars &sc, const float *sec, const float *gamma_set,
   if (o.item_sweeps) o.item_sweeps[item_index] = n_sweeps;
 }
 
-template <typename T>
+template <typename T, typename C = T>
 int bm_run(const orc_bm_scalars *sc, const float *sec, const float *gamma, const orc_bm_item *items, int n_items,
            const float *input, int64_t input_len, const Outs<T> &o, int n_threads) {
   if (!sc || !sec || !gamma || !items || !input || n_items < 0) return -1;
@@ -507,7 +511,7 @@ int bm_run(const orc_bm_scalars *sc, const float *sec, const float *gamma, const
 #pragma omp parallel for schedule(dynamic, 1)
 #endif
   for (int b = 0; b < n_items; ++b)
-    bm_item<T>(*sc, sec, gamma + (int64_t)items[b].set * NS, items[b], b, input, input_len, o);
+    bm_item<T, C>(*sc, sec, gamma + (int64_t)items[b].set * NS, items[b], b, input, input_len, o);
   return 0;
 }
 
@@ -658,6 +662,14 @@ int orc_bm_trace_f32(const orc_bm_scalars *sc, const float *sec, const float *ga
   bm_item<float>(*sc, sec, gamma + (int64_t)item->set * NS, *item, 0, input, input_len, o, &tr);
   *n_rec = tr.n;
   return 0;
+}
+
+int orc_bm_run_f64_clock32(const orc_bm_scalars *sc, const float *sec, const float *gamma, const orc_bm_item *items,
+                           int n_items, const float *input, int64_t input_len, double *speeds, int64_t rows,
+                           int64_t *item_steps, int64_t *item_passes, int64_t *item_sweeps, int n_threads) {
+  Outs<double> o{speeds, rows, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                 item_steps, item_passes, item_sweeps};
+  return bm_run<double, float>(sc, sec, gamma, items, n_items, input, input_len, o, n_threads);
 }
 
 int orc_jacobi(const float *Y, const float *rsm, float *p, int sweeps) {

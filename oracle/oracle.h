@@ -117,6 +117,13 @@ typedef struct {
 int orc_bm_trace_f32(const orc_bm_scalars *sc, const float *sec, const float *gamma, const orc_bm_item *item,
                      const float *input, int64_t input_len, orc_pass_record *rec, int cap, int *n_rec);
 
+/* FP64 rebuild with the sample clock (t, time_step_out, Ts, the interpolation
+ * weight) kept in float as the method has it: separates the clock's rounding
+ * from the state's. */
+int orc_bm_run_f64_clock32(const orc_bm_scalars *sc, const float *sec, const float *gamma, const orc_bm_item *items,
+                           int n_items, const float *input, int64_t input_len, double *speeds, int64_t rows,
+                           int64_t *item_steps, int64_t *item_passes, int64_t *item_sweeps, int n_threads);
+
 /* Jacobi sweeps of the kernel convention (Q7) on one right-hand side. */
 int orc_jacobi(const float *Y, const float *rsm, float *p, int sweeps);
 

@@ -144,9 +144,10 @@ def bm_run(sc: BMScalars, sec: np.ndarray, gamma: np.ndarray, items: np.ndarray,
                                   _p(sweeps), C.c_int(threads))
         out.update(speeds=speeds, failed_nodes=fn, conv_time=ct, conv_loops=cl, conv_jacobi=cj,
                    failed_blocks=fb, conv_time_blocks=ctb, conv_loops_blocks=clb, conv_jacobi_blocks=cjb)
-    else:
+    else:   # "f64": FP64 rebuild; "f64c32": the same with the sample clock in float
         speeds = np.zeros((rows, SECTIONS), np.float64)
-        rc = lib().orc_bm_run_f64(C.byref(sc), _p(sec), _p(gamma), _p(items), C.c_int(n), _p(inp),
+        fn = lib().orc_bm_run_f64 if precision == "f64" else lib().orc_bm_run_f64_clock32
+        rc = fn(C.byref(sc), _p(sec), _p(gamma), _p(items), C.c_int(n), _p(inp),
                                   C.c_int64(inp.size), _p(speeds), C.c_int64(rows), _p(steps), _p(passes),
                                   _p(sweeps), C.c_int(threads))
         out.update(speeds=speeds)
