@@ -29,12 +29,28 @@ __device__ __forceinline__ int fir_window(int64_t k, const cm_ihc_interval &iv, 
 
 constexpr int kRows = 16;   // outputs per thread (register block) = taps per chunk
 
+// Row loaders of the FIR helpers: the value of one section at a row.
+struct GlobalRows {   // rows x 256 in global memory, section s
+  const float *__restrict__ src;
+  int s;
+  __device__ __forceinline__ float operator()(int64_t row) const { return src[row * CM_SECTIONS + s]; }
+};
+constexpr int kRing = 128;   // rows of the dS ring of the fused kernel (a power of two)
+struct RingRows {     // this thread's column of an LDS ring, interval-local rows
+  const float *ring;
+  int stride, col;
+  __device__ __forceinline__ float operator()(int64_t row) const {
+    return ring[(int)(row & (kRing - 1)) * stride + col];
+  }
+};
+
 // General register-blocked FIR over the n <= kRows rows row_first.. of one
 // section: inputs stream from the newest output row down, so each output
 // accumulates fma(b[0],x[k]), fma(b[1],x[k-1]), ... in tap order; a tap
 // outside an output's window contributes an exact zero.  No row past the
 // block or before row 0 is read.
-__device__ __forceinline__ void fir_block_general(const float *__restrict__ src, int64_t row_first, int n, int s,
+template <class Rows>
+__device__ __forceinline__ void fir_block_general(const Rows &src, int64_t row_first, int n,
                                                   const double *__restrict__ b, int taps, const int (&w)[kRows],
                                                   double (&acc)[kRows]) {
 #pragma unroll
@@ -44,7 +60,7 @@ __device__ __forceinline__ void fir_block_general(const float *__restrict__ src,
   for (int jj = 0; jj < span; ++jj) {
     const int64_t row = row_first + top - jj;
     if (row < 0) break;
-    const double x = (double)src[row * CM_SECTIONS + s];
+    const double x = (double)src(row);
 #pragma unroll
     for (int i = 0; i < kRows; ++i) {
       const int t = jj - top + i;
@@ -60,8 +76,11 @@ __device__ __forceinline__ void fir_block_general(const float *__restrict__ src,
 // tap array is zero-padded to a whole number of chunks.  Output i takes tap
 // t from row row_first + i - t, accumulated in ascending t.  Loads are
 // clamped to [0, row_first + n - 1]; clamped rows only meet zero taps or
-// outputs past n.
-__device__ __forceinline__ void fir_block_steady(const float *__restrict__ src, int64_t row_first, int n, int s,
+// outputs past n.  (Requesting the next chunk's rows ahead of the FMAs costs
+// the two-pass kernels a wave per SIMD of occupancy: 114 -> 132 VGPRs, 9.1 ->
+// 19.2 ms on 2.26 M rows.)
+template <class Rows>
+__device__ __forceinline__ void fir_block_steady(const Rows &src, int64_t row_first, int n,
                                                  const double *__restrict__ bpad, int chunks, double (&acc)[kRows]) {
 #pragma unroll
   for (int i = 0; i < kRows; ++i) acc[i] = 0.0;
@@ -71,14 +90,14 @@ __device__ __forceinline__ void fir_block_steady(const float *__restrict__ src, 
   for (int m = kRows - 1; m < 2 * kRows - 1; ++m) {
     int64_t r = row_first - (kRows - 1) + m;
     r = r > hi ? hi : r;
-    X[m] = (double)src[r * CM_SECTIONS + s];
+    X[m] = (double)src(r);
   }
   for (int c = 0; c < chunks; ++c) {
 #pragma unroll
     for (int m = 0; m < kRows; ++m) {
       int64_t r = row_first - (int64_t)kRows * c - (kRows - 1) + m;
       r = r < 0 ? 0 : r;
-      X[m] = (double)src[r * CM_SECTIONS + s];
+      X[m] = (double)src(r);
     }
     const double *bc = bpad + kRows * c;
 #pragma unroll
@@ -138,10 +157,11 @@ __global__ __launch_bounds__(256) void ac_ds_kernel(const float *__restrict__ bm
   } else {
     int w[kRows];
     double acc[kRows];
+    const GlobalRows rows{bm, s};
     if (block_windows(iv, r0, n, nb, w))
-      fir_block_steady(bm, iv.row0 + r0, n, s, bpad, (nb + kRows - 1) / kRows, acc);
+      fir_block_steady(rows, iv.row0 + r0, n, bpad, (nb + kRows - 1) / kRows, acc);
     else
-      fir_block_general(bm, iv.row0 + r0, n, s, b, nb, w, acc);
+      fir_block_general(rows, iv.row0 + r0, n, b, nb, w, acc);
 #pragma unroll
     for (int i = 0; i < kRows; ++i) {
       ac[i] = (float)acc[i];
@@ -213,10 +233,11 @@ __global__ __launch_bounds__(256) void dc_lambda_kernel(LambdaArgs a) {
   }
   int w[kRows];
   double acc[kRows];
+  const GlobalRows rows{a.dS, s};
   if (block_windows(iv, r0, n, a.ndc, w))
-    fir_block_steady(a.dS, iv.row0 + r0, n, s, a.dcpad, (a.ndc + kRows - 1) / kRows, acc);
+    fir_block_steady(rows, iv.row0 + r0, n, a.dcpad, (a.ndc + kRows - 1) / kRows, acc);
   else
-    fir_block_general(a.dS, iv.row0 + r0, n, s, a.dc, a.ndc, w, acc);
+    fir_block_general(rows, iv.row0 + r0, n, a.dc, a.ndc, w, acc);
   const int64_t plane = a.rows * CM_SECTIONS;
 #pragma unroll
   for (int i = 0; i < kRows; ++i) {
@@ -240,10 +261,111 @@ __global__ __launch_bounds__(256) void dc_lambda_kernel(LambdaArgs a) {
   }
 }
 
+// PRE_IHC, IHC and the three lambdas of one node (CudaCalcIHCComposite /
+// CudaCalcLambda order of operations).
+struct LambdaNode {
+  double dmg, zero_factor, A3[3], sp[3], zo[3];
+  __device__ __forceinline__ void init(const LambdaArgs &a, int set, int s) {
+    dmg = a.damage[(int64_t)set * CM_SECTIONS + s];
+    zero_factor = s != 0 ? 1.0 : 0.0;
+    for (int j = 0; j < 3; ++j) {
+      A3[j] = (double)a.A[j * CM_SECTIONS + s];
+      sp[j] = (double)a.spont[j];
+      zo[j] = s != 0 ? 0.0 : sp[j];
+    }
+  }
+  __device__ __forceinline__ void store(const LambdaArgs &a, int64_t o, int64_t plane, float ac, float ds,
+                                        float dc) const {
+    const float pre = __builtin_fmaf(ac, a.etaAC, (float)((double)dc * a.etaDC));
+    const float ihc = (float)log10(fmax((double)(dmg * (double)pre), 0.0) + 1e-20);
+    for (int j = 0; j < 3; ++j) {
+      const double base = fmin(a.sat, fmax(A3[j] * (double)ihc, sp[j]));
+      a.lambda[j * plane + o] = (float)fma(zero_factor, base, zo[j]);
+    }
+    if (a.stages) {
+      a.stages[0 * plane + o] = ac;
+      a.stages[1 * plane + o] = ds;
+      a.stages[2 * plane + o] = dc;
+      a.stages[3 * plane + o] = pre;
+      a.stages[4 * plane + o] = ihc;
+    }
+  }
+};
+
+// The whole chain in one pass for a FIR AC filter: BM -> AC FIR -> dS -> DC
+// box-car FIR -> PRE_IHC -> IHC -> 3 lambdas, with no AC / dS round trip
+// through HBM and no workspace.  A workgroup is SW sections of one tile of
+// kTile rows of an interval; each thread walks its section down the tile in
+// blocks of kRows rows.  dS goes into this thread's column of an LDS ring of
+// kRing rows, from which the DC FIR of the same and later blocks reads it;
+// no thread reads another's column, so the ring needs no barrier.  The DC
+// window reaches up to kHalo rows back, so a tile first computes dS for the
+// kHalo rows before it (within its interval: with kbase 0 no window crosses
+// the interval start).  The arithmetic is the two-kernel path's, so the
+// results are bit-identical.
+constexpr int kTile = 1024;                // output rows per workgroup
+constexpr int kHalo = 96;                  // dS rows before a tile (DC taps <= 97)
+template <int SW>
+__global__ __launch_bounds__(SW) void ihc_fused_kernel(const float *__restrict__ bm, const double *__restrict__ b,
+                                                      const double *__restrict__ bpad, int nb, float coefA,
+                                                      double coefB, LambdaArgs a) {
+  __shared__ float ring[kRing * SW];
+  constexpr int kGroups = CM_SECTIONS / SW;
+  int tx, ty;
+  xcd_tile(tx, ty);
+  const cm_ihc_interval iv = a.ivs[ty];
+  const int tid = threadIdx.x;
+  const int s = (tx % kGroups) * SW + tid;
+  const int64_t r_begin = (int64_t)(tx / kGroups) * kTile;
+  if (r_begin >= iv.len) return;
+  const int64_t r_end = r_begin + kTile < iv.len ? r_begin + kTile : iv.len;
+  // rows before the first DC input are never read unless with a zero tap;
+  // they must hold finite values
+  for (int r = 0; r < kRing; ++r) ring[r * SW + tid] = 0.0f;
+  LambdaNode node;
+  node.init(a, iv.set, s);
+  const GlobalRows bmrows{bm, s};
+  const RingRows dsrows{ring, SW, tid};
+  const int64_t plane = a.rows * CM_SECTIONS;
+  const int ac_chunks = (nb + kRows - 1) / kRows, dc_chunks = (a.ndc + kRows - 1) / kRows;
+  const int64_t start = r_begin > kHalo ? r_begin - kHalo : 0;
+  for (int64_t r0 = start; r0 < r_end; r0 += kRows) {
+    const int n = r_end - r0 < kRows ? (int)(r_end - r0) : kRows;
+    int w[kRows];
+    double acc[kRows];
+    float ac[kRows];
+    if (block_windows(iv, r0, n, nb, w))
+      fir_block_steady(bmrows, iv.row0 + r0, n, bpad, ac_chunks, acc);
+    else
+      fir_block_general(bmrows, iv.row0 + r0, n, b, nb, w, acc);
+#pragma unroll
+    for (int i = 0; i < kRows; ++i) ac[i] = (float)acc[i];
+    float ds[kRows];
+#pragma unroll
+    for (int i = 0; i < kRows; ++i) {
+      if (i < n) {
+        const float mid = __builtin_fmaf(bm[(iv.row0 + r0 + i) * CM_SECTIONS + s], coefA,
+                                         (float)((double)ac[i] * coefB));
+        ds[i] = mid * mid;
+        ring[(int)((r0 + i) & (kRing - 1)) * SW + tid] = ds[i];
+      }
+    }
+    if (r0 + kRows <= r_begin) continue;      // halo block: dS only
+    if (block_windows(iv, r0, n, a.ndc, w))
+      fir_block_steady(dsrows, r0, n, a.dcpad, dc_chunks, acc);
+    else
+      fir_block_general(dsrows, r0, n, a.dc, a.ndc, w, acc);
+#pragma unroll
+    for (int i = 0; i < kRows; ++i)
+      if (i < n) node.store(a, (iv.row0 + r0 + i) * CM_SECTIONS + s, plane, ac[i], ds[i], (float)acc[i]);
+  }
+}
+
 }  // namespace cm
 
-// Runs the whole IHC/lambda chain for one interval table.  scratch must hold
-// 2 x rows x 256 floats (AC, dS).
+// Runs the whole IHC/lambda chain for one interval table.  d_scratch (2 x
+// rows x 256 floats: AC, dS) selects the two-pass kernels; NULL the one-pass
+// kernel (FIR AC filter, kbase 0 intervals, DC taps <= 97).
 extern "C" int cm_launch_ihc_lambda(const float *d_bm, int64_t rows, const cm_ihc_interval *d_ivs,
                                     const cm_ihc_interval *h_ivs, int n_ivs, const cm_ihc_interval *d_iir_ivs,
                                     int n_iir_ivs, const double *d_ac_b, const double *d_ac_bpad,
@@ -256,18 +378,8 @@ extern "C" int cm_launch_ihc_lambda(const float *d_bm, int64_t rows, const cm_ih
   for (int i = 0; i < n_ivs; ++i) max_len = h_ivs[i].len > max_len ? h_ivs[i].len : max_len;
   const int64_t gx = (max_len + kRows - 1) / kRows;
   if (gx > 0x7fffffff || n_ivs > 65535) return -3;
-  float *AC = d_scratch;
-  float *dS = d_scratch + rows * CM_SECTIONS;
   const int iir = d_ac_a != nullptr;
-  if (iir) {
-    hipLaunchKernelGGL(ac_iir_kernel, dim3(n_iir_ivs), dim3(256), 0, stream, d_bm, d_iir_ivs, d_ac_b, d_ac_a, n_ac,
-                       AC);
-  }
-  hipLaunchKernelGGL(ac_ds_kernel, dim3((unsigned)gx, n_ivs), dim3(256), 0, stream, d_bm, d_ivs, d_ac_b, d_ac_bpad,
-                     n_ac, scale_bm, -1.0, iir, AC, dS);
   LambdaArgs a;
-  a.AC = AC;
-  a.dS = dS;
   a.ivs = d_ivs;
   a.dc = d_dc;
   a.dcpad = d_dcpad;
@@ -281,6 +393,35 @@ extern "C" int cm_launch_ihc_lambda(const float *d_bm, int64_t rows, const cm_ih
   a.rows = rows;
   a.lambda = d_lambda;
   a.stages = d_stages;
+  // Two passes through the AC/dS workspace when the caller gives one (faster:
+  // 9.1 against 20.6 ms on 2.26 M rows; the one-pass kernel holds 1.25 waves
+  // per SIMD, its dS ring filling the LDS); one pass, with no workspace and
+  // 16 instead of 32 B of HBM traffic per node, when it does not.  The one-pass
+  // kernel needs every window to start at its interval's first row (kbase 0)
+  // and the DC window to fit the tile halo; an IIR AC filter still needs the
+  // workspace for AC.
+  const bool fused = d_scratch == nullptr;
+  if (fused) {
+    if (n_dc > kHalo + 1 || iir) return -3;
+    for (int i = 0; i < n_ivs; ++i)
+      if (h_ivs[i].kbase != 0) return -3;
+    constexpr int SW = 64;
+    const int64_t tiles = (max_len + kTile - 1) / kTile * (CM_SECTIONS / SW);
+    if (tiles > 0x7fffffff) return -3;
+    hipLaunchKernelGGL((ihc_fused_kernel<SW>), dim3((unsigned)tiles, n_ivs), dim3(SW), 0, stream, d_bm, d_ac_b,
+                       d_ac_bpad, n_ac, scale_bm, -1.0, a);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+  }
+  float *AC = d_scratch;
+  float *dS = d_scratch + rows * CM_SECTIONS;
+  if (iir) {
+    hipLaunchKernelGGL(ac_iir_kernel, dim3(n_iir_ivs), dim3(256), 0, stream, d_bm, d_iir_ivs, d_ac_b, d_ac_a, n_ac,
+                       AC);
+  }
+  hipLaunchKernelGGL(ac_ds_kernel, dim3((unsigned)gx, n_ivs), dim3(256), 0, stream, d_bm, d_ivs, d_ac_b, d_ac_bpad,
+                     n_ac, scale_bm, -1.0, iir, AC, dS);
+  a.AC = AC;
+  a.dS = dS;
   hipLaunchKernelGGL(dc_lambda_kernel, dim3((unsigned)gx, n_ivs), dim3(256), 0, stream, a);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }

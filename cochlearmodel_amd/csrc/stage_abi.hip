@@ -332,9 +332,25 @@ int cm_ihc_lambda_run(const float *d_bm, int64_t rows, const cm_ihc_interval *h_
                       int n_ac, const double *d_dc, int n_dc, const double *d_damage, const float *d_A,
                       const float *spont3, float lambda_sat, float eta_ac, float eta_dc, float scale_bm,
                       float *d_scratch, float *d_lambda, float *d_stages) {
-  if (!d_bm || !h_ivs || n_ivs <= 0 || !d_ac_b || !d_dc || !d_damage || !d_A || !spont3 || !d_scratch || !d_lambda) {
+  if (!d_bm || !h_ivs || n_ivs <= 0 || !d_ac_b || !d_dc || !d_damage || !d_A || !spont3 || !d_lambda) {
     set_error("cm_ihc_lambda_run: null argument");
     return -3;
+  }
+  if (!d_scratch) {   // the one-pass kernel: FIR AC, windows from each interval's first row, DC in the halo
+    if (d_ac_a) {
+      set_error("cm_ihc_lambda_run: an IIR AC filter needs the workspace");
+      return -3;
+    }
+    if (n_dc > 97) {
+      set_error("cm_ihc_lambda_run: %d DC taps need the workspace (one pass holds 97)", n_dc);
+      return -3;
+    }
+    for (int i = 0; i < n_ivs; ++i)
+      if (h_ivs[i].kbase != 0) {
+        set_error("cm_ihc_lambda_run: interval %d continues a window (kbase %d): it needs the workspace", i,
+                  h_ivs[i].kbase);
+        return -3;
+      }
   }
   for (int i = 0; i < n_ivs; ++i)
     if (h_ivs[i].row0 < 0 || h_ivs[i].row0 + h_ivs[i].len > rows || h_ivs[i].len < 0) {

@@ -157,12 +157,13 @@ class BatchedIHC:
         self.device = device
 
     def run(self, d_bm, rows: int, intervals: np.ndarray, iir_intervals=None, d_lambda=None, d_scratch=None,
-            stages=False):
+            stages=False, workspace=True):
+        """workspace=False runs the one-pass kernel (no AC/dS workspace)."""
         torch = self.torch
         dev = self.device
         if d_lambda is None:
             d_lambda = torch.empty((LAMBDA_COUNT, rows, SECTIONS), dtype=torch.float32, device=dev)
-        if d_scratch is None:
+        if d_scratch is None and workspace:
             d_scratch = torch.empty((2, rows, SECTIONS), dtype=torch.float32, device=dev)
         d_st = torch.zeros((5, rows, SECTIONS), dtype=torch.float32, device=dev) if stages else None
         ivs = np.ascontiguousarray(intervals, IHC_INTERVAL_DTYPE)

@@ -182,7 +182,10 @@ void *getCudaLambda(void);
 void *getCudaBuffer(void);
 void *getCudaMeanRate(void);
 /* batched IHC/lambda over device pointers and an interval table (no
- * reference counterpart); d_scratch holds 2 x rows x 256 floats.  With an IIR
+ * reference counterpart); d_scratch holds 2 x rows x 256 floats (AC, dS) for
+ * the two-pass kernels.  d_scratch = NULL runs the chain in one pass with no
+ * workspace (slower, half the HBM traffic): FIR AC filters only, every
+ * interval with kbase 0, at most 97 DC taps.  With an IIR
  * AC filter (d_ac_a) h_iir_ivs lists the runs the recursion restarts on:
  * rows [row0, row0 + len), period len (CudaIIRFilterDecoupled,
  * src/cochlea.cu:2885-2908) */

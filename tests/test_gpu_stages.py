@@ -77,7 +77,38 @@ def test_ihc_batched_signal_intervals(oracle):
     for k, (s0, n) in enumerate(zip(starts, lens)):
         lo = oracle.ihc_lambda(bm[s0:s0 + n], n, n + 1, b, stages.dc_filter(FS), stages.ihc_damage(ihcv[k]),
                                stages.aihc_table(), np.array(stages.SPONT_DEFAULT, np.float32), threads=8)
-        assert _rel(lam[:, s0:s0 + n], lo) <= 1e-4, k
+        np.testing.assert_array_equal(lam[:, s0:s0 + n], lo, err_msg=f"signal {k}")
+
+
+def test_ihc_one_pass_and_two_pass_paths_agree(oracle):
+    """cm_ihc_lambda_run runs AC -> dS -> DC -> lambda in two passes through
+    the caller's AC/dS workspace, or in one pass with no workspace.  Both give
+    the same lambda and stages, equal to the oracle, bit for bit; a table whose
+    second entry continues the first one's windows (kbase > 0) gives the same
+    result through the workspace and is refused without one."""
+    import torch
+    from cochlearmodel_amd import _lib, stages
+    _say("case: IHC/lambda one pass vs two passes over 3000 rows (one and two interval entries), stages kept")
+    rng = np.random.default_rng(21)
+    rows = 3000
+    bm = (1e-3 * rng.standard_normal((rows, 256))).astype(np.float32)
+    d_bm = torch.from_numpy(bm).cuda()
+    bi = stages.BatchedIHC(FS)
+    one = stages.signal_intervals([0], [rows])
+    two = np.concatenate([stages.signal_intervals([0], [1100]), stages.signal_intervals([1100], [rows - 1100])])
+    two["kbase"] = [0, 1100]
+    two["L"], two["tla"], two["frdp"] = rows, rows, rows + 1
+    runs = [bi.run(d_bm, rows, one, stages=True, workspace=ws) for ws in (True, False)]
+    runs.append(bi.run(d_bm, rows, two, stages=True))
+    res = [(lam.cpu().numpy(), st.cpu().numpy()) for lam, st in runs]
+    b, _ = stages.load_ac_filter(FS)
+    lo, so = oracle.ihc_lambda(bm, rows, rows + 1, b, stages.dc_filter(FS), stages.ihc_damage(8.0),
+                               stages.aihc_table(), np.array(stages.SPONT_DEFAULT, np.float32), stages=True, threads=8)
+    for k, (lam, st) in enumerate(res):
+        np.testing.assert_array_equal(lam, lo, err_msg=f"run {k}")
+        np.testing.assert_array_equal(st, so, err_msg=f"run {k}")
+    with pytest.raises(_lib.CochleaError, match="kbase"):
+        bi.run(d_bm, rows, two, workspace=False)
 
 
 def test_bm_batched_sets_and_launch_shapes(oracle):
