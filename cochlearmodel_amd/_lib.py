@@ -106,6 +106,8 @@ _SIGS = {
     "getCudaMeanRate": (vp, []),
     "cm_ihc_lambda_run": (C.c_int, [vp, i64, vp, C.c_int, vp, C.c_int, vp, vp, C.c_int, vp, C.c_int, vp, vp, vp, f32,
                                     f32, f32, f32, vp, vp, vp]),
+    "cm_ihc_lambda_run_noalloc": (C.c_int, [vp, i64, vp, C.c_int, vp, C.c_int, vp, C.c_int, vp, vp, vp, f32, f32, f32,
+                                            f32, vp, vp]),
     "CudaCalculateJND": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, f64, f64, f64, vp, vp, C.c_int, vp, C.c_int,
                                    C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp,
                                    C.c_int, C.c_int, f64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),

@@ -66,6 +66,44 @@ struct StageHolder {
 };
 StageHolder g_st;
 
+// Device scratch of cm_ihc_lambda_run_noalloc: the interval table and the
+// zero-padded AC and DC tap tables, staged in a fixed array of this code
+// object instead of allocated memory.  Every GPU holds its own copy for the
+// life of the process; host code reaches it only through the per-device
+// address from hipGetSymbolAddress, never through the array's name.
+constexpr int kNoallocIntervals = 4096;
+constexpr int kNoallocAcTaps = 512;    // a multiple of 16 (the FIR chunk)
+constexpr int kNoallocDcTaps = 112;    // 97 used at most, padded to 16
+constexpr size_t kNoallocIvBytes = (size_t)kNoallocIntervals * sizeof(cm_ihc_interval);
+constexpr size_t kNoallocBytes =
+    kNoallocIvBytes + (size_t)kNoallocAcTaps * sizeof(double) + (size_t)kNoallocDcTaps * sizeof(double);
+static __device__ __align__(256) unsigned char scratch[kNoallocBytes];
+
+constexpr int kMaxDevices = 64;
+static unsigned char *g_scratch_dev[kMaxDevices];   // per-device address of `scratch`
+
+// The current device's address of the scratch array, looked up once per device.
+static unsigned char *noalloc_scratch() {
+  int dev = 0;
+  if (check(hipGetDevice(&dev), "hipGetDevice")) return nullptr;
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("cm_ihc_lambda_run_noalloc: device %d beyond the %d supported", dev, kMaxDevices);
+    return nullptr;
+  }
+  if (!g_scratch_dev[dev]) {
+    void *p = nullptr;
+    if (check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress")) return nullptr;
+    g_scratch_dev[dev] = static_cast<unsigned char *>(p);
+  }
+  return g_scratch_dev[dev];
+}
+
+// dst[i] = src[i] for i < n, 0 up to npad (a fill kernel rather than a memset)
+__global__ void pad_taps_kernel(double *dst, const double *src, int n, int npad) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < npad) dst[i] = i < n ? src[i] : 0.0;
+}
+
 }  // namespace cm
 
 using namespace cm;
@@ -375,6 +413,53 @@ int cm_ihc_lambda_run(const float *d_bm, int64_t rows, const cm_ihc_interval *h_
                                       g_st.ac_bpad.ptr, d_ac_a, n_ac, d_dc, g_st.dcpad.ptr, n_dc, d_damage, d_A, spont3, lambda_sat, eta_ac, eta_dc,
                                       scale_bm, d_scratch, d_lambda, d_stages, stream());
   if (rc) set_error("cm_ihc_lambda_run: launch failed (%d)", rc);
+  return rc ? -1 : 0;
+}
+
+// cm_ihc_lambda_run without allocating: the one-pass kernel (no AC/dS
+// workspace) with its tables staged in the device scratch array.  FIR AC
+// filters only, at most kNoallocIntervals intervals (kbase 0), kNoallocAcTaps
+// AC taps and 97 DC taps.  Calls that overlap on one GPU (other streams or
+// threads) race on the scratch array.
+int cm_ihc_lambda_run_noalloc(const float *d_bm, int64_t rows, const cm_ihc_interval *h_ivs, int n_ivs,
+                              const double *d_ac_b, int n_ac, const double *d_dc, int n_dc, const double *d_damage,
+                              const float *d_A, const float *spont3, float lambda_sat, float eta_ac, float eta_dc,
+                              float scale_bm, float *d_lambda, float *d_stages) {
+  if (!d_bm || !h_ivs || n_ivs <= 0 || !d_ac_b || !d_dc || !d_damage || !d_A || !spont3 || !d_lambda || rows <= 0) {
+    set_error("cm_ihc_lambda_run_noalloc: null argument");
+    return -3;
+  }
+  if (n_ivs > kNoallocIntervals || n_ac < 1 || n_ac > kNoallocAcTaps || n_dc < 1 || n_dc > 97) {
+    set_error("cm_ihc_lambda_run_noalloc: %d intervals, %d AC and %d DC taps; at most %d, %d and 97", n_ivs, n_ac,
+              n_dc, kNoallocIntervals, kNoallocAcTaps);
+    return -3;
+  }
+  for (int i = 0; i < n_ivs; ++i) {
+    if (h_ivs[i].row0 < 0 || h_ivs[i].len < 0 || h_ivs[i].row0 + h_ivs[i].len > rows) {
+      set_error("cm_ihc_lambda_run_noalloc: interval %d outside [0, %lld)", i, (long long)rows);
+      return -3;
+    }
+    if (h_ivs[i].kbase != 0) {
+      set_error("cm_ihc_lambda_run_noalloc: interval %d continues a window (kbase %d)", i, h_ivs[i].kbase);
+      return -3;
+    }
+  }
+  unsigned char *base = noalloc_scratch();
+  if (!base) return -1;
+  cm_ihc_interval *d_ivs = reinterpret_cast<cm_ihc_interval *>(base);
+  double *ac_pad = reinterpret_cast<double *>(base + kNoallocIvBytes);
+  double *dc_pad = ac_pad + kNoallocAcTaps;
+  const int ac_np = (n_ac + 15) / 16 * 16, dc_np = (n_dc + 15) / 16 * 16;
+  if (check(hipMemcpyAsync(d_ivs, h_ivs, (size_t)n_ivs * sizeof(cm_ihc_interval), hipMemcpyHostToDevice, stream()),
+            "interval table"))
+    return -1;
+  hipLaunchKernelGGL(pad_taps_kernel, dim3((ac_np + 255) / 256), dim3(256), 0, stream(), ac_pad, d_ac_b, n_ac, ac_np);
+  hipLaunchKernelGGL(pad_taps_kernel, dim3(1), dim3(256), 0, stream(), dc_pad, d_dc, n_dc, dc_np);
+  if (check(hipGetLastError(), "pad taps")) return -1;
+  const int rc = cm_launch_ihc_lambda(d_bm, rows, d_ivs, h_ivs, n_ivs, nullptr, 0, d_ac_b, ac_pad, nullptr, n_ac, d_dc,
+                                      dc_pad, n_dc, d_damage, d_A, spont3, lambda_sat, eta_ac, eta_dc, scale_bm,
+                                      nullptr, d_lambda, d_stages, stream());
+  if (rc) set_error("cm_ihc_lambda_run_noalloc: launch failed (%d)", rc);
   return rc ? -1 : 0;
 }
 

@@ -195,6 +195,20 @@ int cm_ihc_lambda_run(const float *d_bm, int64_t rows, const cm_ihc_interval *h_
                       const float *spont3, float lambda_sat, float eta_ac, float eta_dc, float scale_bm,
                       float *d_scratch, float *d_lambda, float *d_stages);
 
+/* cm_ihc_lambda_run without any device allocation (no reference counterpart;
+ * for callers that cannot allocate: code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  The same inputs and outputs with
+ * no workspace: the chain runs in one pass, its interval table and padded tap
+ * tables staged in a fixed device array of the library (133 KiB per GPU, held
+ * until the process ends).  Limits: FIR AC filters, at most 4096 intervals
+ * (kbase 0), 512 AC taps, 97 DC taps; larger requests return -3.  Every call on
+ * one GPU shares that array: calls that overlap on a GPU (several streams or
+ * host threads) race. */
+int cm_ihc_lambda_run_noalloc(const float *d_bm, int64_t rows, const cm_ihc_interval *h_ivs, int n_ivs,
+                              const double *d_ac_b, int n_ac, const double *d_dc, int n_dc, const double *d_damage,
+                              const float *d_A, const float *spont3, float lambda_sat, float eta_ac, float eta_dc,
+                              float scale_bm, float *d_lambda, float *d_stages);
+
 /* ---- JND --------------------------------------------------------------- */
 /* replaces CudaCalculateJND, include/cochlea_common.h:292-328 (src/cochlea.cu:3946-4065);
  * backup_stage 1..6 (AvgMeanRate, dMeanRate, dLambda, preFisherAI,
