@@ -19,7 +19,8 @@ SECTIONS = 256
 
 
 class ToneBatch:
-    def __init__(self, freqs, levels, duration: float = 0.2, with_lambda: bool = True, wn_seed_offset: int = 2):
+    def __init__(self, freqs, levels, duration: float = 0.2, with_lambda: bool = True, wn_seed_offset: int = 2,
+                 d_scratch=None):
         import torch
         self.torch = torch
         self.L = _lib.lib()
@@ -35,13 +36,7 @@ class ToneBatch:
         self.bufP1 = self.ov + self.block * (self.TB + 1)
         self.rows = self.n * self.interval
         self.head = int(round(FS * 0.014))
-        prof = np.zeros(self.n, _lib.PROFILE_DTYPE)
-        prof["dA"] = 10.0 * 10.0 ** (levels / 20.0) * 2e-5
-        prof["recipdA"] = 1.0 / prof["dA"]
-        prof["dBSPLSignal"] = levels
-        prof["frequency"] = freqs
-        prof["referenceIndex"] = -1
-        prof["calculateIndexPosition"] = np.arange(self.n)
+        prof = tone_profiles(freqs, levels)
         self.profiles = prof
         rng = np.random.default_rng(workloads.MASTER_SEED + wn_seed_offset)
         WN = rng.standard_normal(self.interval).astype(np.float32)
@@ -66,13 +61,25 @@ class ToneBatch:
             self.ihc = stages.BatchedIHC(FS)
             self.fir_ivs, _ = stages.reference_intervals(self.rows, self.block, self.TB, self.DF)
             self.d_lambda = torch.empty((3, self.rows, SECTIONS), dtype=torch.float32, device=dev)
-            self.d_scratch = torch.empty((2, self.rows, SECTIONS), dtype=torch.float32, device=dev)
+            # the IHC/lambda AC/dS workspace; batches run one after another may share one
+            if d_scratch is None:
+                d_scratch = torch.empty((2, self.rows, SECTIONS), dtype=torch.float32, device=dev)
+            if d_scratch.numel() < 2 * self.rows * SECTIONS:
+                raise ValueError("d_scratch holds fewer than 2 x rows x 256 floats")
+            self.d_scratch = d_scratch
         self.stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
 
-    def generate_input(self):
+    def device_profiles(self, freqs, levels):
+        """Another set of n tones' profiles, resident on the device, for
+        generate_input(d_prof=...): the same batch geometry, other signals."""
+        assert len(freqs) == self.n and len(levels) == self.n
+        return self.torch.from_numpy(tone_profiles(freqs, levels).view(np.uint8).copy()).to("cuda")
+
+    def generate_input(self, d_prof=None):
         L, lib = self.L, _lib
         L.cm_set_stream(self.stream)
-        lib.check(L.cm_generate_input_run(lib.ptr(self.d_prof), lib.ptr(self.d_WN), None, lib.ptr(self.d_input),
+        d_prof = self.d_prof if d_prof is None else d_prof
+        lib.check(L.cm_generate_input_run(lib.ptr(d_prof), lib.ptr(self.d_WN), None, lib.ptr(self.d_input),
                                           self.norms[0], self.norms[1], 0, self.norms[2], self.norms[3], 0, self.n,
                                           self.interval, self.head, C.c_float(FS)), "input")
 
@@ -115,9 +122,30 @@ class ToneBatch:
         stride = max(1, self.TB // n_blocks)
         return np.arange(0, self.TB, stride)[:n_blocks]
 
+    def sampled_speeds(self, rows):
+        """BM velocity rows (host int array) copied from the device, without
+        copying the rest of the batch."""
+        idx = self.torch.from_numpy(np.asarray(rows, np.int64)).to(self.d_speeds.device)
+        return self.d_speeds.index_select(0, idx).cpu().numpy()
+
     def block_rows(self, sel):
         ws, me = self.items["write_start"][sel], self.items["mesi"][sel]
         return np.concatenate([np.arange(a, b) for a, b in zip(ws, me)])
+
+
+def tone_profiles(freqs, levels) -> np.ndarray:
+    """device_jnd_params of one tone per interval: peak dA = 10 SPLRef 10^(L/20)
+    (Sim_Type 3 profiles without references)."""
+    freqs = np.asarray(freqs, np.float64)
+    levels = np.asarray(levels, np.float64)
+    prof = np.zeros(len(freqs), _lib.PROFILE_DTYPE)
+    prof["dA"] = 10.0 * 10.0 ** (levels / 20.0) * 2e-5
+    prof["recipdA"] = 1.0 / prof["dA"]
+    prof["dBSPLSignal"] = levels
+    prof["frequency"] = freqs
+    prof["referenceIndex"] = -1
+    prof["calculateIndexPosition"] = np.arange(len(freqs))
+    return prof
 
 
 def random_tones(n: int, seed: int, f_lo: float = 250.0, f_hi: float = 8000.0, db_lo: float = 20.0,

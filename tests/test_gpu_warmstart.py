@@ -148,7 +148,9 @@ def test_in_solver_network_matches_oracle(n, dur, per_item, group):
     ref, _ = mo.forward(params, R[:, :769], bf16=True)
     rel = np.linalg.norm(y - ref) / np.linalg.norm(ref)
     _say(f"in-solver network vs oracle: rel L2 {rel:.2e} over {len(R)} records")
-    assert rel <= 1e-2
+    # BF16 operands through the same MFMA tiles: 3-5e-5 measured; the bound
+    # leaves a factor 10 for the hardware exp2/rcp GELU against the oracle's tanh
+    assert rel <= 5e-4
 
 
 def test_trained_network_gains_over_plain_and_extrapolated(oracle):
@@ -192,8 +194,10 @@ def test_trained_network_gains_over_plain_and_extrapolated(oracle):
     err = {k: np.linalg.norm(r[0][rows] - ex) / np.linalg.norm(ex) for k, r in res.items()}
     _say(f"rel L2 vs exact solve: {err}")
     assert res["learned"][1] < res["extrapolated"][1] < res["plain"][1]
-    assert err["learned"] <= max(1.5 * err["plain"], 1e-3)
-    assert err["learned_every2"] <= max(1.5 * err["plain"], 1e-3)
+    # every guess iterates to the same residual tolerance: the errors sit at the
+    # same level (the FP32 method's own floor, 1-3e-4 at these levels)
+    assert err["learned"] <= 1.25 * err["plain"] and err["learned"] <= 5e-4
+    assert err["learned_every2"] <= 1.25 * err["plain"] and err["learned_every2"] <= 5e-4
     assert abs(int(res["learned"][2]) - int(res["plain"][2])) <= 0.01 * res["plain"][2]
 
 
