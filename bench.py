@@ -101,6 +101,16 @@ VALU_ISSUE_MODEL = ("measured: bm_kernel at full occupancy issues 7.38e11 VALU i
 # FP32 without packed-FP32 instructions (the build disables them): one FMA per
 # lane per cycle, half the published vector peak
 PEAK_FP32_UNPACKED_TFLOPS = PEAK_FP32_TFLOPS / 2
+# The same counters on one launch of the held-out set (1024 tones, 10240 blocks;
+# profiles/pmc_traffic_c5_learned.md, profiles/pmc_mix_c5_learned.md)
+C5_MEASURED = {
+    "hbm_bytes": {"learned_tol": (399327.125 * 2.0 + 9062884.78125) * 1024,
+                  "ihc_lambda": (4599822.4375 * 2.0 + 18063360.0 + 9173862.4375 * 2.0 + 27095040.0) * 1024},
+    "valu_insts": {"learned_tol": 6.76117305708e11, "plain_tol": 6.3406044423e11,
+                   "ihc_lambda": 5.610570368e9 + 1.0749237728e10},
+    "l2_read_bytes": {"learned_tol": 2.67213476615e11 * 128},
+    "mfma_insts": {"learned_tol": 3.246648384e10},
+}
 # L1<-L2 reads of the learned kernel: TCP_TCC_READ_REQ_sum x 128 B (a 1 GiB copy
 # reads 8.389e6 requests); profiles/pmc_mix_c2_learned.md
 MEASURED_L2_READ_BYTES = {"learned_tol": 6.6959e10 * 128}
@@ -519,7 +529,8 @@ def c5_accuracy(tbs, per_batch: int, threads: int):
     """Sampled blocks of every batch through the oracle: O2 (exact per-pass
     solve, FP32 state), O2 on the input scaled by 1 + 2^-20 (the FP32
     reproducibility floor of the method) and the FP64 rebuild of the reference
-    method.  Returns per batch the sampled rows and the three velocity sets."""
+    method, with the reference's float sample clock and with a double one.
+    Returns per batch the sampled rows and the four velocity sets."""
     from oracle import oracle as O
     pert = 1.0 + 2.0 ** -20
     res = []
@@ -536,7 +547,9 @@ def c5_accuracy(tbs, per_batch: int, threads: int):
                        threads=threads)["speeds"][rows].astype(np.float64) / pert
         o64 = O.bm_run(O.default_scalars(FS), O.section_params(), gam, it, inp, tb.bufP1, "f64",
                        threads=threads)["speeds"][rows]
-        res.append((rows, o2.astype(np.float64), o2p, o64.astype(np.float64)))
+        o64c = O.bm_run(O.default_scalars(FS), O.section_params(), gam, it, inp, tb.bufP1, "f64c32",
+                        threads=threads)["speeds"][rows]
+        res.append((rows, o2.astype(np.float64), o2p, o64.astype(np.float64), o64c.astype(np.float64)))
     return res
 
 
@@ -622,11 +635,35 @@ def c5_main(args):
                                 "frac": ihc_bytes / (np.mean(ihc_ms) * 1e-3) / 1e9 / PEAK_HBM_GBS,
                                 "fp64_flop_per_launch": rows0 * SECTIONS * 2.0 * (76 + 88)},
     }
+    per_launch_s = bm_sum / args.steps
+    M = C5_MEASURED
+    if args.variant in M["hbm_bytes"]:
+        out["roofline"]["traffic"] = M["hbm_bytes"][args.variant]
+        out["roofline"]["traffic_note"] = ("HBM bytes of one launch (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, calibrated "
+                                           "on a 1 GiB copy); algorithmic: input 36 MB + velocity 9.25 GB")
+    if args.variant in M["valu_insts"]:
+        out["roofline"]["valu_issue"] = {
+            "insts_per_launch": M["valu_insts"][args.variant],
+            "frac": M["valu_insts"][args.variant] / per_launch_s / VALU_ISSUE_PER_S,
+            "capacity_per_s": VALU_ISSUE_PER_S, "capacity_model": VALU_ISSUE_MODEL}
+    out["ihc_lambda_roofline"]["traffic"] = M["hbm_bytes"]["ihc_lambda"]
+    out["ihc_lambda_roofline"]["valu_issue"] = {
+        "insts_per_launch": M["valu_insts"]["ihc_lambda"],
+        "frac": M["valu_insts"]["ihc_lambda"] / (float(np.mean(ihc_ms)) * 1e-3) / VALU_ISSUE_PER_S,
+        "capacity_per_s": VALU_ISSUE_PER_S}
     if mlp_flop:
         mlp = mlp_flop / bm_sum / 1e12
         out["mlp_roofline"] = {"bound": "mfma", "engine": "BF16 MFMA", "achieved": mlp, "peak": PEAK_BF16_TFLOPS,
                                "unit": "TFLOP/s", "frac": mlp / PEAK_BF16_TFLOPS,
-                               "flop_per_launch": mlp_flop / args.steps}
+                               "flop_per_launch": mlp_flop / args.steps,
+                               "executed_flop_per_launch": M["mfma_insts"]["learned_tol"] * 16 * 16 * 32 * 2,
+                               "executed_note": "v_mfma_f32_16x16x32_bf16 count x 16384: 12 block rows padded to 16, "
+                                                "769 inputs padded to 800"}
+        out["roofline"]["l2_read"] = {"bytes_per_launch": M["l2_read_bytes"]["learned_tol"],
+                                      "achieved": M["l2_read_bytes"]["learned_tol"] / per_launch_s / 1e12,
+                                      "unit": "TB/s",
+                                      "note": "L1<-L2 reads (TCP_TCC_READ_REQ x 128 B): the network's 1.606 MB of "
+                                              "BF16 weight fragments streamed once per lockstep round of 12 blocks"}
         out["warm_start_model"] = {k: model_meta[k] for k in ("n_signals", "records", "epochs", "final_mse")}
     threads = min(args.cpu_threads, os.cpu_count() or 1)
     if not args.no_variants:
@@ -634,6 +671,7 @@ def c5_main(args):
         o2 = np.concatenate([a[1] for a in acc])
         floor = _rel(np.concatenate([a[2] for a in acc]), o2)
         o64 = np.concatenate([a[3] for a in acc])
+        o64c = np.concatenate([a[4] for a in acc])
         print(f"# oracle on {len(o2)} sampled rows done; FP32 floor {floor:.2e}", file=sys.stderr, flush=True)
         var = {}
         for name in VARIANTS + EXTRA_VARIANTS:
@@ -650,6 +688,7 @@ def c5_main(args):
             var[name] = {"bm_ms": ms, "bm_section_steps_per_s": SECTIONS * int(st[0]) / (ms * 1e-3),
                          "jacobi_iters_per_step": float(st[2] / st[0]), "passes_per_step": float(st[1] / st[0]),
                          "committed_steps": int(st[0]), "rel_l2_vs_exact_solve": _rel(g, o2),
+                         "rel_l2_vs_fp64_rebuild_float_clock": _rel(g, o64c),
                          "rel_l2_vs_fp64_rebuild": _rel(g, o64)}
             print(f"# variant {name}: {ms:.1f} ms over {nb} launches, {st[2] / st[0]:.2f} sweeps/step, "
                   f"rel L2 vs O2 {var[name]['rel_l2_vs_exact_solve']:.2e}", file=sys.stderr, flush=True)
@@ -662,13 +701,17 @@ def c5_main(args):
             "committed_steps_ratio": lp["committed_steps"] / pl["committed_steps"],
             "rel_l2_vs_exact_solve": {"learned": lp["rel_l2_vs_exact_solve"], "plain": pl["rel_l2_vs_exact_solve"],
                                       "fp32_floor": floor},
+            "rel_l2_vs_fp64_rebuild_float_clock": {"learned": lp["rel_l2_vs_fp64_rebuild_float_clock"],
+                                                   "plain": pl["rel_l2_vs_fp64_rebuild_float_clock"],
+                                                   "O2": _rel(o2, o64c)},
             "rel_l2_vs_fp64_rebuild": {"learned": lp["rel_l2_vs_fp64_rebuild"],
                                        "plain": pl["rel_l2_vs_fp64_rebuild"],
                                        "O2": _rel(o2, o64)}}
         out["accuracy_sample"] = (f"BM velocity of {sum(len(a[0]) for a in acc)} rows "
                                   f"({max(1, args.accuracy_blocks // nb)} strided time blocks of each launch) against "
                                   "the CPU oracle's exact per-pass pressure solve O2 (FP32 state) and its FP64 "
-                                  "rebuild of the reference method; fp32_floor = O2 on the input scaled by "
+                                  "rebuilds of the reference method (FP64 state with the reference's float sample "
+                                  "clock, and FP64 throughout); fp32_floor = O2 on the input scaled by "
                                   "1 + 2^-20 against O2, the run-to-run reproducibility of FP32 evaluation")
     if not args.no_cpu:
         # the oracle has no network: the CPU baseline is the same solver with the

@@ -5,6 +5,7 @@ plain and extrapolated guesses.  No reference counterpart.
 
     python -m cochlearmodel_amd.warmstart train --out cochlearmodel_amd/data/warmstart_mlp.bin
     python -m cochlearmodel_amd.warmstart eval --model cochlearmodel_amd/data/warmstart_mlp.bin
+    python -m cochlearmodel_amd.warmstart features --signals 512 --eval-signals 256
 
 Model file: float32 W1 (769 x 512), b1, W2 (512 x 512), b2, W3 (512 x 256), b3,
 c_out, read by cm_ws_load_file as well; training metadata beside it (.json).
@@ -37,6 +38,7 @@ MODEL_PATH = os.path.join(HERE, "data", "warmstart_mlp.bin")
 FS = 44100.0
 TRAIN_SEED = 20261202 + 51     # training tones
 EVAL_SEED = 20261202 + 52      # held-out tones
+SEC_M = 8                      # row of the section table holding the pressure matrix diagonal (CM_SEC_M)
 
 
 def tol_scalars(tau: float = 1e-6, warm_start: int = 1):
@@ -167,9 +169,119 @@ def evaluate(params, c_out, n_signals=256, batch_signals=256, tau=1e-6, seed=EVA
     return out
 
 
+def _records(n_signals, batch_signals, stride, per_item, tau, seed, log):
+    """Captured predictor-pass records and their raw targets
+    (A^-1 Y - p_lin) / rms(r_lin), plus the m row of the pressure matrix."""
+    import torch
+    R, T, m = [], [], None
+    for b in range(max(1, n_signals // batch_signals)):
+        f, db = random_tones(batch_signals, seed + b)
+        tb = ToneBatch(f, db, 0.2, with_lambda=False)
+        tb.generate_input()
+        rec = capture(tb, tau, stride, per_item)
+        T.append(targets(tb.bm.sec, rec))
+        R.append(rec)
+        if m is None:
+            m = tb.bm.sec.view(-1, 256)[SEC_M].double().clone()
+        log(f"records batch {b}: {rec.shape[0]}")
+        del tb
+        torch.cuda.empty_cache()
+    return torch.cat(R), torch.cat(T), m
+
+
+def _apply_A(m, p):
+    """Rows of A p for the solver's pressure matrix: l_i p_{i-1} + m_i p_i + p_{i+1},
+    l_i = 1 for 0 < i < 254, p_{256} = 0."""
+    import torch
+    out = m * p
+    out[:, :-1] += p[:, 1:]
+    out[:, 1:254] += p[:, 0:253]
+    return out
+
+
+def _smooth(y):
+    """[1,2,1]/4 with replicated edges, as the solver applies to the output."""
+    import torch
+    left = torch.cat([y[:, :1], y[:, :-1]], 1)
+    right = torch.cat([y[:, 1:], y[:, -1:]], 1)
+    return 0.25 * left + 0.5 * y + 0.25 * right
+
+
+def study_features(n_train=512, n_eval=256, batch_signals=256, stride=1024, per_item=32, tau=1e-6, epochs=24,
+                   lr=1e-3, decay=0.85, log=print):
+    """Ablation of the network's inputs and output on the same captured records.
+
+    The configuration names the inputs "displacement, velocity and last
+    pressure plus the input sample" and a pressure guess as the output.  Three
+    networks of that size (769-512-512-256 GELU) are trained with the same
+    schedule and scored on held-out records by how much closer than the
+    extrapolated pressure p_lin their guess lands to the exact solve x* = A^-1 Y:
+
+    * residual-correction (the solver's): d, v, r_lin = Y - A p_lin, s -> the
+      correction (x* - p_lin) / rms(r_lin), applied smoothed;
+    * pressure-correction: d, v, p_lin (the extrapolated last pressure), s -> the
+      same correction;
+    * pressure-absolute: d, v, p_lin, s -> x* / rms(p_lin), the guess itself.
+
+    Every input vector is divided by its RMS over the sections, as in the
+    solver.  Scores per record: e = |g - x*| / |p_lin - x*| and
+    rho = |Y - A g| / |Y - A p_lin|; the solver's residual gate takes a guess
+    only where rho < 1."""
+    import torch
+    Rt, Tt, m = _records(n_train, batch_signals, stride, per_item, tau, TRAIN_SEED, log)
+    Re, Te, _ = _records(n_eval, batch_signals, stride, per_item, tau, EVAL_SEED, log)
+
+    def rms(x):
+        return torch.sqrt((x.double() ** 2).mean(1, keepdim=True)).float() + 1e-30
+
+    def inputs(R, kind):
+        if kind == "residual":
+            return R[:, :SIZES[0]].contiguous()
+        plin = R[:, _lib.WS_REC_PLIN:_lib.WS_REC_PLIN + 256]
+        return torch.cat([R[:, :512], plin / rms(plin), R[:, 768:769]], 1).contiguous()
+
+    def target(R, T, kind):
+        if kind != "absolute":
+            return T
+        plin = R[:, _lib.WS_REC_PLIN:_lib.WS_REC_PLIN + 256]
+        return (plin + R[:, 769:770] * T) / rms(plin)
+
+    out = {}
+    for name, feat, kind in (("residual-correction", "residual", "correction"),
+                             ("pressure-correction", "pressure", "correction"),
+                             ("pressure-absolute", "pressure", "absolute")):
+        X, T = inputs(Rt, feat), target(Rt, Tt, kind)
+        ok = torch.isfinite(X).all(1) & torch.isfinite(T).all(1)
+        X, T = X[ok].contiguous(), T[ok].contiguous()
+        c = float(torch.sqrt((T.double() ** 2).mean()).item())
+        net = WarmStartMLP(seed=1, device="cuda")
+        hist = net.train(X, (T / c).contiguous(), epochs=epochs, batch=1024, lr=lr, decay=decay)
+        Xe = inputs(Re, feat)
+        y = torch.cat([net.forward(Xe[i:i + 4096].contiguous()).clone() for i in range(0, Xe.shape[0], 4096)])
+        plin = Re[:, _lib.WS_REC_PLIN:_lib.WS_REC_PLIN + 256].double()
+        Y = Re[:, _lib.WS_REC_Y:_lib.WS_REC_Y + 256].double()
+        s = Re[:, 769:770].double()
+        xs = plin + s * Te.double()
+        if kind == "correction":
+            g = plin + s * c * _smooth(y.double())
+        else:
+            g = rms(plin).double() * c * y.double()
+        e = (g - xs).norm(dim=1) / (plin - xs).norm(dim=1)
+        rho = (Y - _apply_A(m, g)).norm(dim=1) / (Y - _apply_A(m, plin)).norm(dim=1)
+        good = torch.isfinite(e) & torch.isfinite(rho)
+        e, rho = e[good], rho[good]
+        out[name] = {"train_mse": hist[-1], "median_e": float(e.median()), "median_rho": float(rho.median()),
+                     "frac_rho_below_1": float((rho < 1).double().mean()), "records": int(good.sum())}
+        log(f"{name}: {out[name]}")
+        del net
+        torch.cuda.empty_cache()
+    out["train_records"], out["eval_records"] = int(Rt.shape[0]), int(Re.shape[0])
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("cmd", choices=("train", "eval"))
+    ap.add_argument("cmd", choices=("train", "eval", "features"))
     ap.add_argument("--out", default=MODEL_PATH)
     ap.add_argument("--model", default=MODEL_PATH)
     ap.add_argument("--signals", type=int, default=2048)
@@ -187,6 +299,11 @@ def main(argv=None):
         save_model(args.out, params, c_out, meta)
         log(json.dumps({"saved": args.out, **meta}))
         ev = evaluate(params, c_out, n_signals=args.eval_signals, tau=args.tau, log=log)
+    elif args.cmd == "features":
+        res = study_features(n_train=args.signals, n_eval=args.eval_signals, stride=args.stride, tau=args.tau,
+                             epochs=args.epochs, log=log)
+        print(json.dumps({"feature_study": res}), flush=True)
+        return 0
     else:
         params, c_out, meta = load_model(args.model)
         ev = evaluate(params, c_out, n_signals=args.eval_signals, tau=args.tau, log=log)

@@ -108,6 +108,22 @@ def test_step_tolerance_does_not_move_the_floor(table):
     assert np.all(b[low] > 0.3 * a[low]) and np.all(b[low] < 3 * a[low])
 
 
+def tau_sweep(oracle, taus=(1e-5, 1e-6, 1e-7, 3e-8)):
+    """Plain tolerance-mode Jacobi at each relative residual tau against O2,
+    with the sweeps per step it spends (1000-sweep cap per pass)."""
+    inp, items = _setup(oracle)
+    rows = len(TONES) * N
+    o2 = _run(oracle, inp, items, rows)
+    out = {}
+    for tau in taus:
+        sc = oracle.default_scalars(FS)
+        sc.jacobi_mode, sc.tol_rel, sc.check_every, sc.max_sweeps = 1, tau, 4, 1000
+        o = oracle.bm_run(sc, oracle.section_params(), np.full(256, 0.5, np.float32), items, inp, rows, "f32",
+                          threads=min(8, os.cpu_count() or 1))
+        out[tau] = (_per_tone(o["speeds"].astype(np.float64), o2), float(o["sweeps"].sum() / o["steps"].sum()))
+    return out
+
+
 if __name__ == "__main__":
     sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
     from oracle import oracle as O
@@ -118,3 +134,8 @@ if __name__ == "__main__":
     for tol, d in res.items():
         for k, v in d.items():
             print(f"| {k}, MAX_TOLERANCE {tol:g} | " + " | ".join(f"{x:.2e}" for x in v) + " |")
+    print()
+    print("| plain tau vs O2 | " + " | ".join(f"{f:g} Hz / {db:g} dB" for f, db in TONES) + " | sweeps/step |")
+    print("|---|" + "---|" * (len(TONES) + 1))
+    for tau, (v, sps) in tau_sweep(O).items():
+        print(f"| {tau:g} | " + " | ".join(f"{x:.2e}" for x in v) + f" | {sps:.1f} |")
