@@ -6,10 +6,10 @@ level uniform 20-90 dB SPL, seeds disjoint from the network's 2048 training
 tones), 200 ms each at 44.1 kHz, 256 sections, healthy profiles (OHC 0.5,
 IHC 8), FP32, reference layout (Decouple_Filter=10, 20 ms time blocks), all
 four Allowed_Outputs kept in HBM (BM velocity + lambda high/medium/low).  The
-set is four launches of 1024 tones (10240 time blocks each), all resident.
+set is two launches of 2048 tones (20480 time blocks each), all resident.
 
 One step = profile input synthesis + BM/OHC solve + IHC/lambda of one launch,
-cycling over the four.  Inputs and tables are resident in HBM before timing
+alternating between the two.  Inputs and tables are resident in HBM before timing
 starts.  After the timed steps every solver variant runs on the same resident
 inputs and is scored against the CPU oracle's exact per-pass solve on
 sampled blocks.  Prints one JSON line.
@@ -101,15 +101,16 @@ VALU_ISSUE_MODEL = ("measured: bm_kernel at full occupancy issues 7.38e11 VALU i
 # FP32 without packed-FP32 instructions (the build disables them): one FMA per
 # lane per cycle, half the published vector peak
 PEAK_FP32_UNPACKED_TFLOPS = PEAK_FP32_TFLOPS / 2
-# The same counters on one launch of the held-out set (1024 tones, 10240 blocks;
+# The same counters on one launch of the held-out set (2048 tones, 20480 blocks;
 # profiles/pmc_traffic_c5_learned.md, profiles/pmc_mix_c5_learned.md)
 C5_MEASURED = {
-    "hbm_bytes": {"learned_tol": (399327.125 * 2.0 + 9062884.78125) * 1024,
-                  "ihc_lambda": (4599822.4375 * 2.0 + 18063360.0 + 9173862.4375 * 2.0 + 27095040.0) * 1024},
-    "valu_insts": {"learned_tol": 6.76117305708e11, "plain_tol": 6.3406044423e11,
-                   "ihc_lambda": 5.610570368e9 + 1.0749237728e10},
-    "l2_read_bytes": {"learned_tol": 2.67213476615e11 * 128},
-    "mfma_insts": {"learned_tol": 3.246648384e10},
+    "tones_per_launch": 2048,
+    "hbm_bytes": {"learned_tol": (750991.375 * 2.0 + 18125512.59375) * 1024,
+                  "ihc_lambda": (9209605.875 * 2.0 + 36126720.0 + 18368350.0 * 2.0 + 54190080.0) * 1024},
+    "valu_insts": {"learned_tol": 1.344464110694e12, "plain_tol": 1.269936053668e12,
+                   "ihc_lambda": 1.1221170816e10 + 2.1498514912e10},
+    "l2_read_bytes": {"learned_tol": 5.34115188049e11 * 128},
+    "mfma_insts": {"learned_tol": 6.489495072e10},
 }
 # L1<-L2 reads of the learned kernel: TCP_TCC_READ_REQ_sum x 128 B (a 1 GiB copy
 # reads 8.389e6 requests); profiles/pmc_mix_c2_learned.md
@@ -509,13 +510,17 @@ def c4_main(args):
     print(json.dumps(out))
 
 
-C5_PER_BATCH = 1024    # held-out tones per launch (four seeded draws of 256)
-C5_BATCHES = 4         # 4 x 1024 = the 4096 held-out tones
+# Held-out tones per launch (eight seeded draws of 256) and launches: 2 x 2048
+# = the 4096 held-out tones.  20480 blocks per launch are 1707 lockstep groups
+# of 12 on 256 CUs (6.67 per CU, so the last wave of groups leaves a third of
+# the chip idle for 1/7 of the launch, against 1/4 with launches of 1024).
+C5_PER_BATCH = 2048
+C5_BATCHES = 2
 
 
 def c5_tone_sets(n_batches: int = C5_BATCHES, per: int = C5_PER_BATCH):
     """The held-out evaluation set: draws of 256 tones from seeds EVAL_SEED + 0,
-    1, ... (disjoint from the training seeds), concatenated four to a launch."""
+    1, ... (disjoint from the training seeds), concatenated per launch."""
     from cochlearmodel_amd import batch, warmstart
     sub = per // 256
     sets = []
@@ -560,10 +565,10 @@ def _rel(a, b):
 def c5_main(args):
     """Learned warm start against the plain batched solver on the 4096 held-out
     tones (drawn like the 256-tone grid, seeds disjoint from training), 200 ms
-    at 44.1 kHz, 256 sections.  The four launches of 1024 tones stay resident
-    (profiles, inputs, BM velocity, lambda high/medium/low: about 150 GB of
-    HBM); one step = input synthesis + BM + IHC/lambda of one launch, cycling
-    over the four."""
+    at 44.1 kHz, 256 sections.  The two launches of 2048 tones stay resident
+    (profiles, inputs, BM velocity, lambda high/medium/low: about 185 GB of
+    HBM with the shared IHC workspace); one step = input synthesis + BM +
+    IHC/lambda of one launch, alternating between the two."""
     import torch
     from cochlearmodel_amd import batch
     sets = c5_tone_sets(C5_BATCHES, C5_PER_BATCH)
@@ -636,34 +641,39 @@ def c5_main(args):
                                 "fp64_flop_per_launch": rows0 * SECTIONS * 2.0 * (76 + 88)},
     }
     per_launch_s = bm_sum / args.steps
-    M = C5_MEASURED
+    # counter figures are per launch: only for the launch size they were measured on
+    M = C5_MEASURED if C5_MEASURED["tones_per_launch"] == tbs[0].n else {
+        "hbm_bytes": {}, "valu_insts": {}, "l2_read_bytes": {}, "mfma_insts": {}}
     if args.variant in M["hbm_bytes"]:
         out["roofline"]["traffic"] = M["hbm_bytes"][args.variant]
         out["roofline"]["traffic_note"] = ("HBM bytes of one launch (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, calibrated "
-                                           "on a 1 GiB copy); algorithmic: input 36 MB + velocity 9.25 GB")
+                                           "on a 1 GiB copy); algorithmic: input 72 MB + velocity 18.5 GB")
     if args.variant in M["valu_insts"]:
         out["roofline"]["valu_issue"] = {
             "insts_per_launch": M["valu_insts"][args.variant],
             "frac": M["valu_insts"][args.variant] / per_launch_s / VALU_ISSUE_PER_S,
             "capacity_per_s": VALU_ISSUE_PER_S, "capacity_model": VALU_ISSUE_MODEL}
-    out["ihc_lambda_roofline"]["traffic"] = M["hbm_bytes"]["ihc_lambda"]
-    out["ihc_lambda_roofline"]["valu_issue"] = {
-        "insts_per_launch": M["valu_insts"]["ihc_lambda"],
-        "frac": M["valu_insts"]["ihc_lambda"] / (float(np.mean(ihc_ms)) * 1e-3) / VALU_ISSUE_PER_S,
-        "capacity_per_s": VALU_ISSUE_PER_S}
+    if "ihc_lambda" in M["hbm_bytes"]:
+        out["ihc_lambda_roofline"]["traffic"] = M["hbm_bytes"]["ihc_lambda"]
+        out["ihc_lambda_roofline"]["valu_issue"] = {
+            "insts_per_launch": M["valu_insts"]["ihc_lambda"],
+            "frac": M["valu_insts"]["ihc_lambda"] / (float(np.mean(ihc_ms)) * 1e-3) / VALU_ISSUE_PER_S,
+            "capacity_per_s": VALU_ISSUE_PER_S}
     if mlp_flop:
         mlp = mlp_flop / bm_sum / 1e12
         out["mlp_roofline"] = {"bound": "mfma", "engine": "BF16 MFMA", "achieved": mlp, "peak": PEAK_BF16_TFLOPS,
                                "unit": "TFLOP/s", "frac": mlp / PEAK_BF16_TFLOPS,
-                               "flop_per_launch": mlp_flop / args.steps,
-                               "executed_flop_per_launch": M["mfma_insts"]["learned_tol"] * 16 * 16 * 32 * 2,
-                               "executed_note": "v_mfma_f32_16x16x32_bf16 count x 16384: 12 block rows padded to 16, "
-                                                "769 inputs padded to 800"}
-        out["roofline"]["l2_read"] = {"bytes_per_launch": M["l2_read_bytes"]["learned_tol"],
-                                      "achieved": M["l2_read_bytes"]["learned_tol"] / per_launch_s / 1e12,
-                                      "unit": "TB/s",
-                                      "note": "L1<-L2 reads (TCP_TCC_READ_REQ x 128 B): the network's 1.606 MB of "
-                                              "BF16 weight fragments streamed once per lockstep round of 12 blocks"}
+                               "flop_per_launch": mlp_flop / args.steps}
+        if "learned_tol" in M["mfma_insts"]:
+            out["mlp_roofline"]["executed_flop_per_launch"] = M["mfma_insts"]["learned_tol"] * 16 * 16 * 32 * 2
+            out["mlp_roofline"]["executed_note"] = ("v_mfma_f32_16x16x32_bf16 count x 16384: 12 block rows padded to "
+                                                    "16, 769 inputs padded to 800")
+            out["roofline"]["l2_read"] = {"bytes_per_launch": M["l2_read_bytes"]["learned_tol"],
+                                          "achieved": M["l2_read_bytes"]["learned_tol"] / per_launch_s / 1e12,
+                                          "unit": "TB/s",
+                                          "note": "L1<-L2 reads (TCP_TCC_READ_REQ x 128 B): the network's 1.606 MB "
+                                                  "of BF16 weight fragments streamed once per lockstep round of 12 "
+                                                  "blocks"}
         out["warm_start_model"] = {k: model_meta[k] for k in ("n_signals", "records", "epochs", "final_mse")}
     threads = min(args.cpu_threads, os.cpu_count() or 1)
     if not args.no_variants:
