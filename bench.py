@@ -279,7 +279,8 @@ def c3_main(args):
     steps, passes, sweeps = ub.stats()
     ms_per_step = 1e3 * wall / args.steps
     bm_avg = float(np.mean(bm_ms))
-    # the same launch without the tuning: one wave per workgroup, natural order
+    # the same launch without the tuning: one wave per workgroup, natural order,
+    # sweeps in pairs
     ub.clear_order()
     ms_untuned = ub.run_bm(sc, waves_per_block=1)
     flops = flops_per_launch(steps, passes, sweeps)
@@ -297,6 +298,7 @@ def c3_main(args):
         "bm_ms": bm_avg, "ihc_lambda_ms": float(np.mean(lam_ms)), "committed_steps": steps,
         "jacobi_iters_per_step": sweeps / steps, "signals_per_s": ub.n / (ms_per_step * 1e-3),
         "autotune": {"waves_per_block": int(tune["waves_per_block"]), "longest_first": bool(tune["order_lpt"]),
+                     "jacobi_unroll": int(tune["jacobi_unroll"]),
                      "trial_blocks": int(tune["n_trial_items"]), "trial_ms": [float(x) for x in tune["ms"]],
                      "seconds": t_tune, "bm_ms_untuned": ms_untuned},
         "roofline": {"bound": "valu", "engine": "FP32 VALU", "achieved": flops / (bm_avg * 1e-3) / 1e12,

@@ -130,14 +130,34 @@ __device__ __forceinline__ void jacobi_sweep(const float (&Y)[4], const float (&
   q[2] = (Y[2] - __builtin_fmaf(low23, p[1], p[3])) * rsm[2];
   q[3] = (Y[3] - (low23 * p[2] + right)) * rsm[3];
 }
-// J is wave-uniform: the count goes to a scalar register, and sweeps are
-// paired so the two iterates alternate between register sets instead of
-// being copied back each sweep.
+// J is wave-uniform: the count goes to a scalar register.  U (1, 2 or 4) is
+// the unroll of the sweep loop, a launch knob of the autotuner: with U >= 2
+// sweeps come in pairs so the two iterates alternate between register sets
+// instead of being copied back each sweep; U = 4 runs two pairs per loop trip.
+// Every U performs the same sweeps in the same order, so results are identical.
+template <int U = 2>
 __device__ __forceinline__ void jacobi_sweeps(const float (&Y)[4], const float (&rsm)[4], float low23, float (&p)[4],
                                               int J) {
+  static_assert(U == 1 || U == 2 || U == 4, "unroll 1, 2 or 4");
   J = __builtin_amdgcn_readfirstlane(J);
   float q[4];
-  for (int k = 1; k < J; k += 2) {
+  if (U == 1) {
+    for (int k = 0; k < J; ++k) {
+      jacobi_sweep(Y, rsm, low23, p, q);
+      UNROLL for (int j = 0; j < 4; ++j) p[j] = q[j];
+    }
+    return;
+  }
+  int k = 0;
+  if (U == 4) {
+    for (; k + 4 <= J; k += 4) {
+      jacobi_sweep(Y, rsm, low23, p, q);
+      jacobi_sweep(Y, rsm, low23, q, p);
+      jacobi_sweep(Y, rsm, low23, p, q);
+      jacobi_sweep(Y, rsm, low23, q, p);
+    }
+  }
+  for (; k + 2 <= J; k += 2) {
     jacobi_sweep(Y, rsm, low23, p, q);
     jacobi_sweep(Y, rsm, low23, q, p);
   }
@@ -321,6 +341,7 @@ __device__ __forceinline__ float pass_rhs(const ItemConst &k, const ItemScal &c,
 // Jacobi until |Y - A p|_2 <= tol |Y|_2 (checked every check_every sweeps,
 // capped at max_sweeps); returns the sweeps done.  rr is |Y - A p|_2^2 of the
 // incoming p (the guess selection has already computed it).
+template <int U = 2>
 __device__ __forceinline__ int jacobi_tol(const cm_bm_scalars &sc, const ItemConst &k, const float (&Y)[4],
                                           float yy, float (&p)[4], float rr) {
   const float tau2 = sc.tol_rel * sc.tol_rel;
@@ -328,7 +349,7 @@ __device__ __forceinline__ int jacobi_tol(const cm_bm_scalars &sc, const ItemCon
   for (;;) {
     if (rr <= tau2 * yy || J >= sc.max_sweeps) break;
     const int kk = sc.check_every < sc.max_sweeps - J ? sc.check_every : sc.max_sweeps - J;
-    jacobi_sweeps(Y, k.rsm, k.low23, p, kk);
+    jacobi_sweeps<U>(Y, k.rsm, k.low23, p, kk);
     J += kk;
     rr = resid2(Y, k.mf, k.low23, p);
   }
@@ -472,7 +493,8 @@ __device__ __forceinline__ float extrapolate_guess(const ItemConst &k, const flo
 // One time block, include/cochlea.cuh:13620-14232.  TOL selects the
 // sweep-to-tolerance Jacobi; otherwise the reference's fixed 20/2 sweeps.
 // WS = 1 adds the extrapolated predictor-pass guess (tolerance mode only).
-template <bool TOL, int WS>
+// U is the Jacobi sweep unroll.
+template <bool TOL, int WS, int U>
 __device__ void run_item(const BmLaunch &a, int item_index, int lane, float4 *pold) {
   const cm_bm_item it = a.items[item_index];
   ItemConst k;
@@ -505,10 +527,10 @@ __device__ void run_item(const BmLaunch &a, int item_index, int lane, float4 *po
         } else {
           rr = resid2(Y, k.mf, k.low23, s.p);
         }
-        J = jacobi_tol(a.sc, k, Y, yy, s.p, rr);
+        J = jacobi_tol<U>(a.sc, k, Y, yy, s.p, rr);
       } else {
         J = out_loop < 2 ? a.sc.j_slow : a.sc.j_fast;
-        jacobi_sweeps(Y, k.rsm, k.low23, s.p, J);
+        jacobi_sweeps<U>(Y, k.rsm, k.low23, s.p, J);
       }
       s.n_sweeps += J;
       pass_finish(k, c, s, od, G, sf);
@@ -519,7 +541,7 @@ __device__ void run_item(const BmLaunch &a, int item_index, int lane, float4 *po
   item_final(a, s, item_index, lane);
 }
 
-template <int WAVES, bool TOL, int WS>
+template <int WAVES, bool TOL, int WS, int U>
 __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(3))) void bm_kernel(BmLaunch a) {
   const int lane = threadIdx.x & 63;
   // wave-uniform by construction: readfirstlane lets the compiler keep the
@@ -528,7 +550,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(3)))
   const int idx = blockIdx.x * WAVES + wave;
   __shared__ float4 pold_lds[WS ? WAVES * 65 : 1];
   if (idx < a.n_items)
-    run_item<TOL, WS>(a, __builtin_amdgcn_readfirstlane(a.order ? a.order[idx] : idx), lane,
+    run_item<TOL, WS, U>(a, __builtin_amdgcn_readfirstlane(a.order ? a.order[idx] : idx), lane,
                       pold_lds + (WS ? wave * 65 : 0));
 }
 
@@ -879,27 +901,38 @@ __global__ __launch_bounds__(64 * G) void bm_ws_kernel(BmLaunch a, WsArgs w) {
 
 }  // namespace cm
 
-// Launcher used by the C-ABI layer.
-extern "C" int cm_launch_bm(const cm::BmLaunch *a, int waves_per_block, int grid_blocks, hipStream_t stream) {
+// Launcher used by the C-ABI layer: waves per workgroup 1, 2 or 4, Jacobi
+// unroll 1, 2 or 4.
+extern "C" int cm_launch_bm(const cm::BmLaunch *a, int waves_per_block, int unroll, int grid_blocks,
+                            hipStream_t stream) {
   if (a->use_queue) return -1;   // the work-queue launch is switched off
   dim3 grid(grid_blocks);
   const bool tol = a->sc.jacobi_mode == 1;
   const int ws = tol ? a->sc.warm_start : 0;
   if (ws < 0 || ws > 1) return -1;
-#define CM_BM_CASE(W, T, S) \
-  case (W) * 4 + (T) * 2 + (S): \
-    hipLaunchKernelGGL((cm::bm_kernel<W, T, S>), grid, dim3(64 * (W)), 0, stream, *a); \
+  const int u = unroll == 1 ? 0 : (unroll == 2 ? 1 : (unroll == 4 ? 2 : -1));
+  const int w = waves_per_block == 1 ? 0 : (waves_per_block == 2 ? 1 : (waves_per_block == 4 ? 2 : -1));
+  if (u < 0 || w < 0) return -1;
+  const int mode = tol ? 1 + ws : 0;   // 0 fixed sweeps, 1 tolerance, 2 tolerance + extrapolation
+#define CM_BM_CASE(W, U) \
+  case ((W) == 1 ? 0 : ((W) == 2 ? 1 : 2)) * 3 + ((U) == 1 ? 0 : ((U) == 2 ? 1 : 2)): \
+    if (mode == 0) \
+      hipLaunchKernelGGL((cm::bm_kernel<W, false, 0, U>), grid, dim3(64 * (W)), 0, stream, *a); \
+    else if (mode == 1) \
+      hipLaunchKernelGGL((cm::bm_kernel<W, true, 0, U>), grid, dim3(64 * (W)), 0, stream, *a); \
+    else \
+      hipLaunchKernelGGL((cm::bm_kernel<W, true, 1, U>), grid, dim3(64 * (W)), 0, stream, *a); \
     break;
-  switch (waves_per_block * 4 + (tol ? 2 : 0) + ws) {
-    CM_BM_CASE(1, false, 0)
-    CM_BM_CASE(1, true, 0)
-    CM_BM_CASE(1, true, 1)
-    CM_BM_CASE(2, false, 0)
-    CM_BM_CASE(2, true, 0)
-    CM_BM_CASE(2, true, 1)
-    CM_BM_CASE(4, false, 0)
-    CM_BM_CASE(4, true, 0)
-    CM_BM_CASE(4, true, 1)
+  switch (w * 3 + u) {
+    CM_BM_CASE(1, 1)
+    CM_BM_CASE(1, 2)
+    CM_BM_CASE(1, 4)
+    CM_BM_CASE(2, 1)
+    CM_BM_CASE(2, 2)
+    CM_BM_CASE(2, 4)
+    CM_BM_CASE(4, 1)
+    CM_BM_CASE(4, 2)
+    CM_BM_CASE(4, 4)
     default: return -1;
   }
 #undef CM_BM_CASE

@@ -71,8 +71,9 @@ typedef struct {
   int32_t waves_per_block;   // 1, 2 or 4 time blocks per workgroup
   int32_t order_lpt;         // 1: items launched longest first
   int32_t n_trial_items;     // items in the timed sample
-  int32_t pad;
-  float ms[6];               // trial times: (wpb 1, 2, 4) x (natural, longest first)
+  int32_t jacobi_unroll;     // 1, 2 or 4 sweeps per trip of the Jacobi loop
+  float ms[18];              // trial times, index 6 u + 2 w + o: unroll (2, 4, 1) x wpb (1, 2, 4) x
+                             // (natural, longest first)
 } cm_bm_tuning;
 
 // A run of rows of the IHC/lambda stage sharing one FIR window rule:

@@ -71,6 +71,7 @@ struct BMHolder {
   float last_kernel_ms = 0.f;
   int waves_per_block = 1;
   int persistent = 0;
+  int jacobi_unroll = 2;   // Jacobi sweep loop unroll of bm_kernel launches (cm_set_bm_jacobi_unroll)
   // solver options for BMOHCNewKernel launches (cm_set_bm_solver); defaults = reference method
   int jacobi_mode = 0, check_every = 4, max_sweeps = 1000, warm_start = 0;
   float tol_rel = 1e-4f;
@@ -145,7 +146,7 @@ int launch_bm_items(const cm_bm_scalars &sc, const float *d_input, int64_t input
                     const float *d_gamma, const cm_bm_item *d_items, int n_items, float *d_speeds, int64_t rows,
                     int32_t *failed_nodes, float *ct, float *cl, float *cj, int32_t *fb, float *ctb, float *clb,
                     float *cjb, cm_item_stats *d_stats, int waves_per_block, int persistent, float *kernel_ms,
-                    const int32_t *order_override) {
+                    const int32_t *order_override, int unroll = 0) {
   BmLaunch a;
   a.sc = sc;
   a.input = d_input;
@@ -200,7 +201,7 @@ int launch_bm_items(const cm_bm_scalars &sc, const float *d_input, int64_t input
     rc = cm_launch_bm_ws(&a, &w, device_cus(), stream());
     rc = rc > 0 ? 0 : rc;
   } else {
-    rc = cm_launch_bm(&a, waves_per_block, grid, stream());
+    rc = cm_launch_bm(&a, waves_per_block, unroll > 0 ? unroll : g_bm.jacobi_unroll, grid, stream());
   }
   if (rc) {
     set_error("BM kernel launch failed (%d): %s", rc, hipGetErrorString(hipGetLastError()));
@@ -342,15 +343,18 @@ int cm_bm_autotune(const cm_bm_scalars *sc, const float *d_input, int64_t input_
     return -1;
   const int saved_n = g_bm.order_n;
   g_bm.order_n = 0;
-  const int wpb[3] = {1, 2, 4};
+  // trial c = 6 u + 2 w + o: waves per workgroup wpb[w], order o (natural,
+  // longest first), Jacobi unroll unr[u]
+  const int wpb[3] = {1, 2, 4}, unr[3] = {2, 4, 1};
   float best = 1e30f;
   int bi = 0;
-  for (int c = -1; c < 6; ++c) {   // c = -1: untimed warm-up
-    const int w = wpb[(c < 0 ? 0 : c) / 2], o = (c < 0 ? 0 : c) % 2;
+  for (int c = -1; c < 18; ++c) {   // c = -1: untimed warm-up
+    const int cc = c < 0 ? 0 : c;
+    const int u = unr[cc / 6], w = wpb[(cc % 6) / 2], o = cc % 2;
     float ms = 0.f;
     const int rc = launch_bm_items(*sc, d_input, input_len, d_sec, d_gamma, g_bm.trial_items.ptr, ns, d_speeds, rows,
                                    nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
-                                   w, 0, &ms, g_bm.trial_order.ptr + o * ns);
+                                   w, 0, &ms, g_bm.trial_order.ptr + o * ns, u);
     if (rc) {
       g_bm.order_n = saved_n;
       return rc;
@@ -363,10 +367,11 @@ int cm_bm_autotune(const cm_bm_scalars *sc, const float *d_input, int64_t input_
     }
   }
   g_bm.order_n = saved_n;
-  out->waves_per_block = wpb[bi / 2];
+  out->waves_per_block = wpb[(bi % 6) / 2];
   out->order_lpt = bi % 2;
   out->n_trial_items = ns;
-  out->pad = 0;
+  out->jacobi_unroll = unr[bi / 6];
+  g_bm.jacobi_unroll = out->jacobi_unroll;
   if (out->order_lpt) return cm_set_bm_order(full.data(), n_items);
   g_bm.order_n = 0;
   return 0;
@@ -384,6 +389,15 @@ int cm_set_bm_solver(int jacobi_mode, float tol_rel, int check_every, int max_sw
   g_bm.check_every = check_every;
   g_bm.max_sweeps = max_sweeps;
   g_bm.warm_start = warm_start;
+  return 0;
+}
+
+int cm_set_bm_jacobi_unroll(int unroll) {
+  if (unroll != 1 && unroll != 2 && unroll != 4) {
+    set_error("cm_set_bm_jacobi_unroll: unroll must be 1, 2 or 4 (got %d)", unroll);
+    return -3;
+  }
+  g_bm.jacobi_unroll = unroll;
   return 0;
 }
 

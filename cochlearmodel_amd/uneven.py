@@ -80,7 +80,9 @@ class UnevenBatch:
         self.stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
 
     def autotune(self, sc, trial_stride: int = 4):
-        """Pick waves per workgroup and launch order on a sample of the blocks."""
+        """Pick the Jacobi unroll, waves per workgroup and launch order on a
+        sample of the blocks; the library keeps the unroll and order for the
+        following launches."""
         L = lib()
         L.cm_set_stream(self.stream)
         out = np.zeros(1, TUNING_DTYPE)
@@ -92,7 +94,9 @@ class UnevenBatch:
         return out[0]
 
     def clear_order(self):
+        """Back to the untuned launch order and Jacobi unroll."""
         check(lib().cm_set_bm_order(None, 0), "cm_set_bm_order")
+        check(lib().cm_set_bm_jacobi_unroll(2), "cm_set_bm_jacobi_unroll")
 
     def run_bm(self, sc, timed: bool = True, waves_per_block: int | None = None):
         L = lib()

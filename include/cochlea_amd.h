@@ -60,6 +60,11 @@ int cudaEventsDestroy(void *start, void *stop);
 float cm_last_kernel_ms(void);
 /* BM launch shape: waves (time blocks) per workgroup, persistent queue. */
 int cm_set_bm_launch(int waves_per_block, int persistent);
+/* Jacobi sweep loop unroll (1, 2 or 4; default 2) of the following bm_kernel
+ * launches (no reference counterpart: the reference's sweep loop is
+ * include/cochlea.cuh:13962-13982).  Results are identical for every value;
+ * cm_bm_autotune sets the fastest. */
+int cm_set_bm_jacobi_unroll(int unroll);
 /* Solver of the BMOHCNewKernel launches (no reference counterpart): jacobi_mode
  * 0 = the reference's fixed sweeps (default), 1 = Jacobi to relative residual
  * tol_rel checked every check_every sweeps, capped at max_sweeps; warm_start 0
@@ -69,10 +74,11 @@ int cm_set_bm_solver(int jacobi_mode, float tol_rel, int check_every, int max_sw
  * integrates block h_order[w] (a permutation; results do not depend on it).
  * NULL clears it. */
 int cm_set_bm_order(const int32_t *h_order, int n);
-/* Times the launch configurations (waves per workgroup x natural or
- * longest-first order) on every trial_stride-th block of the table (taken in
- * longest-first order) and reports the fastest; installs the longest-first
- * order when it wins.  Results are identical for every configuration.
+/* Times the launch configurations (Jacobi unroll x waves per workgroup x
+ * natural or longest-first order) on every trial_stride-th block of the table
+ * (taken in longest-first order) and reports the fastest; installs its unroll,
+ * and the longest-first order when it wins.  Results are identical for every
+ * configuration.
  * h_items: the block table on the host; d_speeds receives the trial rows. */
 int cm_bm_autotune(const cm_bm_scalars *sc, const float *d_input, int64_t input_len, const float *d_sec,
                    const float *d_gamma, const cm_bm_item *h_items, int n_items, float *d_speeds, int64_t rows,
