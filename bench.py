@@ -98,6 +98,8 @@ MEASURED_VALU_INSTS = {"reference_20_2": 1.0995e11, "learned_tol": 1.6927e11,
 VALU_ISSUE_PER_S = 7.38e11
 VALU_ISSUE_MODEL = ("measured: bm_kernel at full occupancy issues 7.38e11 VALU instructions/s "
                     "(profiles/pmc_valu_c4_reference.md); the 4-cycle wave64 model gives 6.14e11/s")
+# L2 -> L1 fill of all CUs: 64 B per clock per CU, 256 CUs at 2.4 GHz
+L1_FILL_BYTES_PER_S = 64 * 256 * 2.4e9
 # FP32 without packed-FP32 instructions (the build disables them): one FMA per
 # lane per cycle, half the published vector peak
 PEAK_FP32_UNPACKED_TFLOPS = PEAK_FP32_TFLOPS / 2
@@ -676,6 +678,16 @@ def c5_main(args):
                                           "note": "L1<-L2 reads (TCP_TCC_READ_REQ x 128 B): the network's 1.606 MB "
                                                   "of BF16 weight fragments streamed once per lockstep round of 12 "
                                                   "blocks"}
+            # the two limits the lockstep round alternates between, back to back:
+            # VALU issue (solver) and L2->L1 fill (weight stream)
+            valu_s = M["valu_insts"]["learned_tol"] / VALU_ISSUE_PER_S
+            fill_s = M["l2_read_bytes"]["learned_tol"] / L1_FILL_BYTES_PER_S
+            out["roofline"]["serial_limits"] = {
+                "valu_issue_s": valu_s, "l1_fill_s": fill_s, "kernel_s": per_launch_s,
+                "frac": (valu_s + fill_s) / per_launch_s,
+                "note": "VALU instructions at the measured full-occupancy issue rate plus L1<-L2 bytes at 64 B per "
+                        "clock per CU, over the kernel time: the round runs the solver and the network one after "
+                        "the other, so their sum bounds it"}
         out["warm_start_model"] = {k: model_meta[k] for k in ("n_signals", "records", "epochs", "final_mse")}
     threads = min(args.cpu_threads, os.cpu_count() or 1)
     if not args.no_variants:
