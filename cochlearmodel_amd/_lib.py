@@ -73,6 +73,7 @@ _SIGS = {
     "cm_bm_run": (C.c_int, [vp, vp, i64, vp, vp, vp, C.c_int, vp, i64] + [vp] * 9 + [C.c_int, C.c_int, C.c_int]),
     "cm_get_last_blocks": (C.c_int, [vp, C.c_int]),
     "cm_get_last_stats": (C.c_int, [vp, C.c_int]),
+    "BMOHCKernel_Wait_Threads": (C.c_int, []),
     "BMOHCKernel_Copy_Results": (C.c_int, [vp, sz, sz]),
     "ReverseKernel_Copy_Results": (C.c_int, [vp, sz]),
     "extractConvergenceTimes": (C.c_int, [vp, sz]),

@@ -655,6 +655,10 @@ int cm_get_last_stats(cm_item_stats *out, int max_blocks) {
   return n;
 }
 
+// include/cochlea_common.h:179 -> src/cochlea.cu:819-822: waits for all work
+// on the device.
+int BMOHCKernel_Wait_Threads(void) { return check(hipDeviceSynchronize(), "BMOHCKernel_Wait_Threads"); }
+
 // include/cochlea_common.h -> src/cochlea.cu:823-827.
 int BMOHCKernel_Copy_Results(float *target, size_t resultNodes, size_t offset) {
   if (offset + resultNodes > g_bm.speeds.cap) {

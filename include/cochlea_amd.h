@@ -118,6 +118,9 @@ int cm_bm_run(const cm_bm_scalars *sc, const float *d_input, int64_t input_len, 
 int cm_get_last_blocks(cm_bm_item *out, int max_blocks);
 int cm_get_last_stats(cm_item_stats *out, int max_blocks);
 
+/* replaces BMOHCKernel_Wait_Threads, include/cochlea_common.h:179
+ * (src/cochlea.cu:819-822): device-wide synchronisation */
+int BMOHCKernel_Wait_Threads(void);
 /* replaces BMOHCKernel_Copy_Results (src/cochlea.cu:823-827) */
 int BMOHCKernel_Copy_Results(float *target, size_t resultNodes, size_t offset);
 /* replaces ReverseKernel_Copy_Results (src/cochlea.cu:828-833) */

@@ -16,6 +16,8 @@ def test_c1_voice_parity(oracle):
     lay = solver.voice_layout(44100.0, 5, 0.02, 0.005)
     assert (lay.block_nodes, lay.overlap_nodes, lay.bufP1) == (882, 220, 5512)
     g = solver.run_bm_voice(sig, lay)
+    from cochlearmodel_amd import _lib
+    assert _lib.lib().BMOHCKernel_Wait_Threads() == 0
     # oracle on the same inputs
     sc = oracle.default_scalars(44100.0)
     st, mo, me, ws = oracle.block_decomposition(5, lay.bufP1, lay.overlap_nodes, 0)
