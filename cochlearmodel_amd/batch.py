@@ -113,6 +113,15 @@ class ToneBatch:
         t["ihc_ms"] = e2.elapsed_time(e3)
         return t
 
+    def longest_first_order(self) -> np.ndarray:
+        """Launch order of the block table, longest block first (ties in table
+        order): the first block of every tone is the shortest, so in natural
+        order each lockstep group mixes lengths and the last wave of work
+        holds full-length blocks.  For cm_set_bm_order; results do not depend
+        on it."""
+        lens = self.items["mesi"].astype(np.int64) - self.items["start"]
+        return np.argsort(-lens, kind="stable").astype(np.int32)
+
     def stats(self):
         s = self.d_stats.view(self.TB, 3).cpu().numpy().astype(np.int64)
         return int(s[:, 0].sum()), int(s[:, 1].sum()), int(s[:, 2].sum())
