@@ -370,7 +370,13 @@ int cm_ihc_lambda_run(const float *d_bm, int64_t rows, const cm_ihc_interval *h_
                       int n_ac, const double *d_dc, int n_dc, const double *d_damage, const float *d_A,
                       const float *spont3, float lambda_sat, float eta_ac, float eta_dc, float scale_bm,
                       float *d_scratch, float *d_lambda, float *d_stages) {
-  if (!d_bm || !h_ivs || n_ivs <= 0 || !d_ac_b || !d_dc || !d_damage || !d_A || !spont3 || !d_lambda) {
+  if (n_ivs < 0 || n_iir_ivs < 0 || rows < 0) {
+    set_error("cm_ihc_lambda_run: negative count (%d intervals, %d IIR intervals, %lld rows)", n_ivs, n_iir_ivs,
+              (long long)rows);
+    return -3;
+  }
+  if (n_ivs == 0) return 0;   // an empty batch
+  if (!d_bm || !h_ivs || !d_ac_b || !d_dc || !d_damage || !d_A || !spont3 || !d_lambda) {
     set_error("cm_ihc_lambda_run: null argument");
     return -3;
   }
@@ -425,7 +431,12 @@ int cm_ihc_lambda_run_noalloc(const float *d_bm, int64_t rows, const cm_ihc_inte
                               const double *d_ac_b, int n_ac, const double *d_dc, int n_dc, const double *d_damage,
                               const float *d_A, const float *spont3, float lambda_sat, float eta_ac, float eta_dc,
                               float scale_bm, float *d_lambda, float *d_stages) {
-  if (!d_bm || !h_ivs || n_ivs <= 0 || !d_ac_b || !d_dc || !d_damage || !d_A || !spont3 || !d_lambda || rows <= 0) {
+  if (n_ivs < 0 || rows < 0) {
+    set_error("cm_ihc_lambda_run_noalloc: negative count (%d intervals, %lld rows)", n_ivs, (long long)rows);
+    return -3;
+  }
+  if (n_ivs == 0) return 0;   // an empty batch
+  if (!d_bm || !h_ivs || !d_ac_b || !d_dc || !d_damage || !d_A || !spont3 || !d_lambda) {
     set_error("cm_ihc_lambda_run_noalloc: null argument");
     return -3;
   }
@@ -647,7 +658,13 @@ int cm_jnd_run(const float *d_lambda, int64_t rows, int n_intervals, int interva
     set_error("cm_jnd_run: null argument");
     return -3;
   }
-  if ((int64_t)n_intervals * interval_nodes > rows || head_nodes + length_nodes > interval_nodes) {
+  if (n_intervals < 0 || interval_nodes < 0 || head_nodes < 0 || length_nodes < 1 || out_stride < 0 || rows < 0 ||
+      !(Fs > 0.0)) {
+    set_error("cm_jnd_run: negative or empty size (%d intervals of %d nodes, head %d, length %d, stride %d, Fs %g)",
+              n_intervals, interval_nodes, head_nodes, length_nodes, out_stride, Fs);
+    return -3;
+  }
+  if ((int64_t)n_intervals * interval_nodes > rows || (int64_t)head_nodes + length_nodes > interval_nodes) {
     set_error("cm_jnd_run: intervals exceed the lambda planes");
     return -3;
   }
@@ -903,6 +920,11 @@ int cm_generate_input_run(const cm_jnd_profile *d_profiles, const float *d_WN, c
                           int start_profile, int n_profiles, int interval_len, int head, float Fs) {
   if (!d_profiles || !d_WN || !d_out || (signal_mode && !d_Signal)) {
     set_error("cm_generate_input_run: null argument");
+    return -3;
+  }
+  if (start_profile < 0 || n_profiles < 0 || interval_len < 0 || !(Fs > 0.f)) {
+    set_error("cm_generate_input_run: negative size (profiles %d..+%d, interval %d) or Fs %g", start_profile,
+              n_profiles, interval_len, (double)Fs);
     return -3;
   }
   return cm_launch_generate_input(d_profiles, d_WN, d_Signal, d_out, wn_dc, wn_norm, signal_mode, sig_dc, sig_norm,

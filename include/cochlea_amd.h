@@ -197,7 +197,8 @@ void *getCudaMeanRate(void);
  * interval with kbase 0, at most 97 DC taps.  With an IIR
  * AC filter (d_ac_a) h_iir_ivs lists the runs the recursion restarts on:
  * rows [row0, row0 + len), period len (CudaIIRFilterDecoupled,
- * src/cochlea.cu:2885-2908) */
+ * src/cochlea.cu:2885-2908).  n_ivs = 0 is a no-op; negative counts and
+ * intervals outside [0, rows) return -3 before any device call. */
 int cm_ihc_lambda_run(const float *d_bm, int64_t rows, const cm_ihc_interval *h_ivs, int n_ivs,
                       const cm_ihc_interval *h_iir_ivs, int n_iir_ivs, const double *d_ac_b, const double *d_ac_a,
                       int n_ac, const double *d_dc, int n_dc, const double *d_damage, const float *d_A,
@@ -234,7 +235,11 @@ int CudaCalculateJND(int calculate_ai, int calculate_rms, int mean_size, int fis
                      float *FisherAISum, int writeMatrixSize, int calculatedMatrixSize,
                      double JND_Delta_Alpha_Length_Factor, int JND_USE_Spont_Base_Reference_Lambda,
                      int Show_Run_Time, int calcdA, int show_generated_input_params_cuda, int backup_stage);
-/* batched JND over device pointers (no reference counterpart) */
+/* batched JND over device pointers (no reference counterpart).  Refuses (-3,
+ * before any device call) negative sizes, an empty window (length_nodes < 1),
+ * Fs <= 0, and intervals or windows beyond the lambda planes; n_intervals = 0
+ * is a no-op.  d_ref_index / d_out_index are device arrays whose values the
+ * caller keeps within the intervals and the output columns. */
 int cm_jnd_run(const float *d_lambda, int64_t rows, int n_intervals, int interval_nodes, int head_nodes,
                int length_nodes, const float *d_dA, const int32_t *d_ref_index, const int32_t *d_out_index,
                int out_stride, const double *d_nIHC, const float *W3, double Fs, double delta_alpha_factor,
@@ -316,7 +321,10 @@ int generateInputFromProfile(const cm_jnd_profile *profiles, const float *WN, in
                              float Tolerance_Fix_Factor, int Blocks_Per_Interval, int Show_Run_Time,
                              const float *fir_transfer_function, int fir_transfer_function_length,
                              const float *iir_transfer_function, int iir_transfer_function_length);
-/* batched input synthesis over device pointers (no reference counterpart) */
+/* batched input synthesis over device pointers (no reference counterpart):
+ * d_out[p * interval_len + n] for profile start_profile + p, p < n_profiles.
+ * Negative sizes or Fs <= 0 return -3 before any device call; an empty batch
+ * is a no-op. */
 int cm_generate_input_run(const cm_jnd_profile *d_profiles, const float *d_WN, const float *d_Signal, float *d_out,
                           double wn_dc, double wn_norm, int signal_mode, double sig_dc, double sig_norm,
                           int start_profile, int n_profiles, int interval_len, int head, float Fs);

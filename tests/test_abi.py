@@ -82,3 +82,50 @@ def test_bm_run_rejects_invalid_arguments_before_any_launch():
         bad.jacobi_mode, bad.warm_start, bad.check_every, bad.tol_rel, bad.max_sweeps = 1, ws, check_every, 1e-6, 1000
         assert run(bad) == -3 and "warm_start" in msg(), (ws, check_every)
     assert run(sc, n_items=0) == 0
+
+
+def test_batched_stage_entry_points_reject_bad_sizes_before_any_launch():
+    """The batched IHC/lambda, JND and input-synthesis entry points refuse
+    negative sizes (and an empty JND window, Fs <= 0) with -3 and a message,
+    and take an empty batch as a no-op; every case returns before a device
+    call, so this runs without a GPU."""
+    import ctypes as C
+    from cochlearmodel_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built")
+    L = _lib.lib()
+    f = C.c_void_p(0x1000)   # never dereferenced
+    sp = (C.c_float * 3)(0.0, 0.0, 0.0)
+    W = (C.c_float * 3)(1.0, 1.0, 1.0)
+    msg = lambda: L.cm_last_error().decode()
+
+    def ihc(rows, n_ivs, n_iir=0):
+        return L.cm_ihc_lambda_run(f, rows, f, n_ivs, None, n_iir, f, None, 76, f, 88, f, f, sp, C.c_float(500),
+                                   C.c_float(1), C.c_float(100), C.c_float(1), f, f, None)
+
+    assert ihc(100, -1) == -3 and "negative count" in msg()
+    assert ihc(-5, 1) == -3 and "negative count" in msg()
+    assert ihc(100, 1, n_iir=-2) == -3
+    assert ihc(100, 0) == 0
+    assert L.cm_ihc_lambda_run_noalloc(f, 100, f, -1, f, 76, f, 88, f, f, sp, C.c_float(500), C.c_float(1),
+                                       C.c_float(100), C.c_float(1), f, None) == -3
+    assert L.cm_ihc_lambda_run_noalloc(f, 100, None, 0, f, 76, f, 88, f, f, sp, C.c_float(500), C.c_float(1),
+                                       C.c_float(100), C.c_float(1), f, None) == 0
+
+    def jnd(n, nodes, head, length, stride=1, rows=10000, fs=44100.0):
+        return L.cm_jnd_run(f, rows, n, nodes, head, length, f, f, f, stride, f, W, fs, 0.2 * fs, 1, 1, f, f, f)
+
+    for args in ((-1, 100, 0, 50), (2, -100, 0, 50), (2, 100, -1, 50), (2, 100, 0, 0), (2, 100, 0, 50, -1)):
+        assert jnd(*args) == -3, args
+    assert jnd(2, 100, 0, 50, fs=0.0) == -3
+    assert jnd(200, 100, 0, 50) == -3 and "exceed" in msg()
+    assert jnd(2, 100, 60, 50) == -3 and "exceed" in msg()
+    assert jnd(0, 100, 0, 50) == 0
+
+    def gen(start, n, length, fs=44100.0):
+        return L.cm_generate_input_run(f, f, None, f, 0.0, 1.0, 0, 0.0, 1.0, start, n, length, 0, C.c_float(fs))
+
+    for args in ((-1, 2, 100), (0, -2, 100), (0, 2, -100)):
+        assert gen(*args) == -3 and "negative size" in msg(), args
+    assert gen(0, 2, 100, fs=0.0) == -3
+    assert gen(0, 0, 100) == 0
