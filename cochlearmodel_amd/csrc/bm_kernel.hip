@@ -424,7 +424,7 @@ __device__ __forceinline__ void step_commit(const BmLaunch &a, const cm_bm_item 
     if (s.in_s + 1 >= it.mosi) {
       const float div_times = s.c_dt / s.c_steps;
       const float div_j = s.c_passes / s.c_steps;
-      if (s.in_s >= it.write_start && s.in_s < a.rows) {
+      if (s.in_s >= it.write_start && s.in_s >= 0 && s.in_s < a.rows) {   // rows outside the buffer are dropped
         if (a.speeds)
           *reinterpret_cast<float4 *>(a.speeds + (int64_t)s.in_s * CM_SECTIONS + 4 * lane) =
               make_float4(s.nv[0], s.nv[1], s.nv[2], s.nv[3]);

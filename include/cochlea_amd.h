@@ -109,7 +109,10 @@ int BMOHCNewKernel(const float *input_samples, int override_input_samples, float
 /* Batched BM launch over an explicit block table (device pointers).  Rows of
  * d_speeds are input-sample indices; any per-row / per-block output may be
  * NULL.  No reference counterpart: the reference runs one parameter set per
- * launch (src/solver.cpp:312). */
+ * launch (src/solver.cpp:312).  The kernel reads input only inside
+ * [0, input_len) (zero elsewhere) and writes rows only inside [0, rows); each
+ * item's set must index a row of d_gamma (the table is device memory, which
+ * the host does not read). */
 int cm_bm_run(const cm_bm_scalars *sc, const float *d_input, int64_t input_len, const float *d_sec,
               const float *d_gamma, const cm_bm_item *d_items, int n_items, float *d_speeds, int64_t rows,
               int32_t *d_failed_nodes, float *d_conv_time, float *d_conv_loops, float *d_conv_jacobi,
