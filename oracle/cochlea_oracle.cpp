@@ -447,7 +447,7 @@ void bm_item(const orc_bm_scalars &sc, const float *sec, const float *gamma_set,
         r.step = (int32_t)n_steps; r.pass = pass; r.in_s = in_s; r.decided = decided;
         r.t = (float)t; r.dt_in = (float)dt_in; r.dt_out = (float)dt;
         r.L = (float)d.L; r.F = (float)d.fsum; r.e_a = (float)d.emax; r.e_v = (float)d.esum;
-        r.another = another; r.reset = reset; r.capped = capped;
+        r.another = another; r.reset = reset; r.capped = capped; r.sweeps = J;
       }
 #if ORC_MUTANT == 6
       if (another) continue;   // restart the step from the predictor with the halved dt

@@ -111,6 +111,7 @@ typedef struct {
   float dt_in, dt_out;
   float L, F, e_a, e_v;
   int32_t another, reset, capped;   /* after the decision and the pass cap */
+  int32_t sweeps;    /* Jacobi sweeps of the pass (0 for the exact solve) */
 } orc_pass_record;
 /* Kernel-17 run of one item that records its passes (up to cap; the run stops
  * once the record is full). */

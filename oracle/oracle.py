@@ -158,7 +158,7 @@ def bm_run(sc: BMScalars, sec: np.ndarray, gamma: np.ndarray, items: np.ndarray,
 PASS_RECORD_DTYPE = np.dtype([("step", np.int32), ("pass_", np.int32), ("in_s", np.int32), ("decided", np.int32),
                               ("t", np.float32), ("dt_in", np.float32), ("dt_out", np.float32), ("L", np.float32),
                               ("F", np.float32), ("e_a", np.float32), ("e_v", np.float32), ("another", np.int32),
-                              ("reset", np.int32), ("capped", np.int32)])
+                              ("reset", np.int32), ("capped", np.int32), ("sweeps", np.int32)])
 
 
 def step_control(aref, na, vref, nv, dt: float, tol: float, m1tol: float, dt_min: float = 1e-18,
