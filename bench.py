@@ -107,11 +107,11 @@ PEAK_FP32_UNPACKED_TFLOPS = PEAK_FP32_TFLOPS / 2
 # longest block first; profiles/pmc_traffic_c5_learned.md, profiles/pmc_mix_c5_learned.md)
 C5_MEASURED = {
     "tones_per_launch": 2048,
-    "hbm_bytes": {"learned_tol": (917211.75 * 2.0 + 18125642.875) * 1024,
+    "hbm_bytes": {"learned_tol": (920172.375 * 2.0 + 18125642.5) * 1024,
                   "ihc_lambda": (9206258.75 * 2.0 + 36126720.0 + 18361109.0625 * 2.0 + 54190080.0) * 1024},
-    "valu_insts": {"learned_tol": 1.336624513254e12, "plain_tol": 1.269936053668e12,
+    "valu_insts": {"learned_tol": 1.310250600478e12, "plain_tol": 1.269936053668e12,
                    "ihc_lambda": 1.1221170816e10 + 2.1498514912e10},
-    "l2_read_bytes": {"learned_tol": 5.23537335271e11 * 128},
+    "l2_read_bytes": {"learned_tol": 5.23537354631e11 * 128},
     "mfma_insts": {"learned_tol": 6.360480032e10},
 }
 # L1<-L2 reads of the learned kernel: TCP_TCC_READ_REQ_sum x 128 B (a 1 GiB copy

@@ -5,7 +5,7 @@ plain and extrapolated guesses.  No reference counterpart.
 
     python -m cochlearmodel_amd.warmstart train --out cochlearmodel_amd/data/warmstart_mlp.bin
     python -m cochlearmodel_amd.warmstart eval --model cochlearmodel_amd/data/warmstart_mlp.bin
-    python -m cochlearmodel_amd.warmstart features --signals 512 --eval-signals 256
+    python -m cochlearmodel_amd.warmstart features --signals 512 --eval-signals 256 --stride 1024 --per-item 32
 
 Model file: float32 W1 (769 x 512), b1, W2 (512 x 512), b2, W3 (512 x 256), b3,
 c_out, read by cm_ws_load_file as well; training metadata beside it (.json).
@@ -36,8 +36,12 @@ from .mlp import SIZES, WarmStartMLP
 HERE = os.path.dirname(os.path.abspath(__file__))
 MODEL_PATH = os.path.join(HERE, "data", "warmstart_mlp.bin")
 FS = 44100.0
-TRAIN_SEED = 20261202 + 51     # training tones
-EVAL_SEED = 20261202 + 52      # held-out tones
+# Batch b of 256 tones is drawn with seed TRAIN_SEED + b (training) or
+# EVAL_SEED + b (held out).  The two ranges are a million seeds apart, so no
+# batch is ever in both sets (checked by tests/test_warmstart_seeds.py).
+TRAIN_SEED = 20261202 + 1_000_000   # training tones
+EVAL_SEED = 20261202 + 52           # held-out tones
+SEED_RANGE = 1_000_000              # batches available to each set
 SEC_M = 8                      # row of the section table holding the pressure matrix diagonal (CM_SEC_M)
 
 
@@ -118,8 +122,11 @@ def build_training_set(n_signals: int, batch_signals: int, stride: int, per_item
     return torch.cat(Xs), torch.cat(Ts)
 
 
-def train(n_signals=2048, batch_signals=256, stride=1024, per_item=32, tau=1e-6, epochs=24, lr=1e-3, decay=0.85,
+def train(n_signals=2048, batch_signals=256, stride=64, per_item=512, tau=1e-6, epochs=24, lr=1e-3, decay=0.85,
           seed=TRAIN_SEED, log=print):
+    """2048 solver trajectories (tones drawn like the 256-tone grid), a record of
+    every 64th step of every time block (7.6 M records, 23 GB of features on
+    the device), 24 epochs of Adam on the library's BF16-MFMA kernels."""
     import torch
     t0 = time.time()
     X, T = build_training_set(n_signals, batch_signals, stride, per_item, tau, seed, log)
@@ -132,7 +139,8 @@ def train(n_signals=2048, batch_signals=256, stride=1024, per_item=32, tau=1e-6,
     t1 = time.time()
     hist = net.train(X, T, epochs=epochs, batch=1024, lr=lr, decay=decay, log=log)
     torch.cuda.synchronize()
-    meta = {"n_signals": n_signals, "records": int(X.shape[0]), "stride": stride, "tau": tau, "epochs": epochs,
+    meta = {"n_signals": n_signals, "records": int(X.shape[0]), "stride": stride, "per_item": per_item, "tau": tau,
+            "epochs": epochs,
             "lr": lr, "decay": decay, "seed": seed, "final_mse": hist[-1], "train_s": time.time() - t1,
             "features": "d, v, r_lin (RMS-normalised), s / rms(p_prev)",
             "target": "(A^-1 Y - p_lin) / (rms(r_lin) c_out)"}
@@ -287,21 +295,22 @@ def main(argv=None):
     ap.add_argument("--signals", type=int, default=2048)
     ap.add_argument("--eval-signals", type=int, default=256)
     ap.add_argument("--epochs", type=int, default=24)
-    ap.add_argument("--stride", type=int, default=1024)
+    ap.add_argument("--stride", type=int, default=64, help="steps between captured records of a block")
+    ap.add_argument("--per-item", type=int, default=512, help="record slots per block (>= steps per block / stride)")
     ap.add_argument("--tau", type=float, default=1e-6)
     args = ap.parse_args(argv)
     import torch
     torch.cuda.set_device(0)
     log = lambda m: print(m, flush=True)
     if args.cmd == "train":
-        params, c_out, meta, hist = train(n_signals=args.signals, stride=args.stride, tau=args.tau,
-                                          epochs=args.epochs, log=log)
+        params, c_out, meta, hist = train(n_signals=args.signals, stride=args.stride, per_item=args.per_item,
+                                          tau=args.tau, epochs=args.epochs, log=log)
         save_model(args.out, params, c_out, meta)
         log(json.dumps({"saved": args.out, **meta}))
         ev = evaluate(params, c_out, n_signals=args.eval_signals, tau=args.tau, log=log)
     elif args.cmd == "features":
-        res = study_features(n_train=args.signals, n_eval=args.eval_signals, stride=args.stride, tau=args.tau,
-                             epochs=args.epochs, log=log)
+        res = study_features(n_train=args.signals, n_eval=args.eval_signals, stride=args.stride,
+                             per_item=args.per_item, tau=args.tau, epochs=args.epochs, log=log)
         print(json.dumps({"feature_study": res}), flush=True)
         return 0
     else:
