@@ -79,16 +79,16 @@ MLP_FLOP_PER_STEP = 2.0 * (769 * 512 + 512 * 512 + 512 * 256)   # forward, one b
 # WRITE_SIZE (KiB per dispatch), scaled by what the counters read for a 1 GiB
 # device copy (FETCH x 2.0, WRITE x 1.0); profiles/pmc_traffic_c2.md
 MEASURED_HBM_BYTES = {
-    "learned_tol": (50009.375 * 2.0 + 2265904.0) * 1024,
-    "reference_20_2": (6083.44 * 2.0 + 2258260.0) * 1024,
-    "ihc_lambda": (1150705.0 * 2.0 + 4515840.0 + 2289528.0 * 2.0 + 6773760.0) * 1024,
+    "learned_tol": (47479.375 * 2.0 + 2265904.0) * 1024,
+    "reference_20_2": (6085.375 * 2.0 + 2258260.0) * 1024,
+    "ihc_lambda": (1150991.3125 * 2.0 + 4515840.0 + 2288920.5 * 2.0 + 6773760.0) * 1024,
 }
 PEAK_BF16_TFLOPS = 2500.0    # MI355X dense BF16 MFMA
 # VALU instructions per launch on the 256-tone batch (rocprofv3 --pmc SQ_INSTS_VALU;
 # profiles/pmc_valu_c2_reference.md, profiles/pmc_mix_c2_learned.md) and the issue
 # capacity they are measured against: a wave64 FP32 VALU instruction holds a SIMD
 # for 4 cycles, 1024 SIMDs at 2.4 GHz
-MEASURED_VALU_INSTS = {"reference_20_2": 1.0995e11, "learned_tol": 1.6927e11,
+MEASURED_VALU_INSTS = {"reference_20_2": 1.0995e11, "learned_tol": 1.6547e11,
                        "ihc_lambda": 1.40262e9 + 2.68728e9}   # ac_ds_kernel + dc_lambda_kernel
 # VALU issue capacity: the rate bm_kernel's own instruction mix reaches at full
 # occupancy (SQ_INSTS_VALU / kernel time on the JND sweep's 48300 blocks,
