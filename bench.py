@@ -299,9 +299,11 @@ def c3_main(args):
                    "signals": ub.n, "time_blocks": ub.TB, "rows": ub.rows, "samples": int(sum(ub.lengths))},
         "bm_ms": bm_avg, "ihc_lambda_ms": float(np.mean(lam_ms)), "committed_steps": steps,
         "jacobi_iters_per_step": sweeps / steps, "signals_per_s": ub.n / (ms_per_step * 1e-3),
-        "autotune": {"waves_per_block": int(tune["waves_per_block"]), "longest_first": bool(tune["order_lpt"]),
+        "autotune": {"sections_per_lane": int(tune["sections_per_lane"]),
+                     "waves_per_block": int(tune["waves_per_block"]), "longest_first": bool(tune["order_lpt"]),
                      "jacobi_unroll": int(tune["jacobi_unroll"]),
                      "trial_blocks": int(tune["n_trial_items"]), "trial_ms": [float(x) for x in tune["ms"]],
+                     "trial_ms_split": [float(x) for x in tune["ms_split"]],
                      "seconds": t_tune, "bm_ms_untuned": ms_untuned},
         "roofline": {"bound": "valu", "engine": "FP32 VALU", "achieved": flops / (bm_avg * 1e-3) / 1e12,
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",

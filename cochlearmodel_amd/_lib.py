@@ -32,8 +32,10 @@ ITEM_DTYPE = np.dtype([("start", np.int32), ("mosi", np.int32), ("mesi", np.int3
                        ("set", np.int32), ("pad", np.int32)])
 STATS_DTYPE = np.dtype([("steps", np.int64), ("passes", np.int64), ("sweeps", np.int64)])
 TUNING_DTYPE = np.dtype([("waves_per_block", np.int32), ("order_lpt", np.int32), ("n_trial_items", np.int32),
-                         ("jacobi_unroll", np.int32), ("ms", np.float32, 18)])
-TUNING_UNROLLS = (2, 4, 1)   # trial index 6 u + 2 w + o
+                         ("jacobi_unroll", np.int32), ("ms", np.float32, 18), ("sections_per_lane", np.int32),
+                         ("ms_split", np.float32, 12)])
+TUNING_UNROLLS = (2, 4, 1)   # ms index 6 u + 2 w + o; ms_split index 6 v + 2 u + o
+TUNING_SPLITS = (2, 1)       # sections per lane of ms_split's v
 IHC_INTERVAL_DTYPE = np.dtype([("row0", np.int64), ("len", np.int32), ("kbase", np.int32), ("L", np.int32),
                                ("tla", np.int32), ("frdp", np.int32), ("set", np.int32)])
 assert IHC_INTERVAL_DTYPE.itemsize == 32
@@ -62,6 +64,7 @@ _SIGS = {
     "cm_last_kernel_ms": (C.c_float, []),
     "cm_set_bm_launch": (C.c_int, [C.c_int, C.c_int]),
     "cm_set_bm_jacobi_unroll": (C.c_int, [C.c_int]),
+    "cm_set_bm_sections_per_lane": (C.c_int, [C.c_int]),
     "cm_set_bm_order": (C.c_int, [vp, C.c_int]),
     "cm_set_bm_solver": (C.c_int, [C.c_int, f32, C.c_int, C.c_int, C.c_int]),
     "cm_bm_autotune": (C.c_int, [vp, vp, i64, vp, vp, vp, C.c_int, vp, i64, C.c_int, vp]),

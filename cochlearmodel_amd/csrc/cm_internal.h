@@ -115,8 +115,10 @@ int bm_ensure_input(size_t n);
 
 }  // namespace cm
 
-extern "C" int cm_launch_bm(const cm::BmLaunch *a, int waves_per_block, int unroll, int grid_blocks,
-                            hipStream_t stream);
+extern "C" int cm_launch_bm(const cm::BmLaunch *a, int waves_per_block, int unroll, int sections_per_lane,
+                            int grid_blocks, hipStream_t stream);
+// a block over a workgroup of 2 or 4 waves (sections_per_lane 2 or 1)
+extern "C" int cm_launch_bm_split(const cm::BmLaunch *a, int sections_per_lane, int unroll, hipStream_t stream);
 // lockstep-group launch of the warm-start kernel; returns the group size used
 extern "C" int cm_launch_bm_ws(const cm::BmLaunch *a, const cm::WsArgs *w, int n_cus, hipStream_t stream);
 extern "C" int cm_launch_ihc_lambda(const float *d_bm, int64_t rows, const cm_ihc_interval *d_ivs,

@@ -72,8 +72,11 @@ typedef struct {
   int32_t order_lpt;         // 1: items launched longest first
   int32_t n_trial_items;     // items in the timed sample
   int32_t jacobi_unroll;     // 1, 2 or 4 sweeps per trip of the Jacobi loop
-  float ms[18];              // trial times, index 6 u + 2 w + o: unroll (2, 4, 1) x wpb (1, 2, 4) x
-                             // (natural, longest first)
+  float ms[18];              // trial times, 4 sections per lane, index 6 u + 2 w + o: unroll (2, 4, 1) x
+                             // wpb (1, 2, 4) x (natural, longest first)
+  int32_t sections_per_lane; // 4 (one wave per block), 2 or 1 (a block over 2 or 4 waves)
+  float ms_split[12];        // trial times of 2 and 1 sections per lane, index 6 v + 2 u + o:
+                             // (2, 1) x unroll (2, 4, 1) x (natural, longest first)
 } cm_bm_tuning;
 
 // A run of rows of the IHC/lambda stage sharing one FIR window rule:

@@ -72,6 +72,7 @@ struct BMHolder {
   int waves_per_block = 1;
   int persistent = 0;
   int jacobi_unroll = 2;   // Jacobi sweep loop unroll of bm_kernel launches (cm_set_bm_jacobi_unroll)
+  int sections_per_lane = 4;   // block layout of fused-kernel launches (cm_set_bm_sections_per_lane)
   // solver options for BMOHCNewKernel launches (cm_set_bm_solver); defaults = reference method
   int jacobi_mode = 0, check_every = 4, max_sweeps = 1000, warm_start = 0;
   float tol_rel = 1e-4f;
@@ -146,7 +147,7 @@ int launch_bm_items(const cm_bm_scalars &sc, const float *d_input, int64_t input
                     const float *d_gamma, const cm_bm_item *d_items, int n_items, float *d_speeds, int64_t rows,
                     int32_t *failed_nodes, float *ct, float *cl, float *cj, int32_t *fb, float *ctb, float *clb,
                     float *cjb, cm_item_stats *d_stats, int waves_per_block, int persistent, float *kernel_ms,
-                    const int32_t *order_override, int unroll = 0) {
+                    const int32_t *order_override, int unroll = 0, int sections_per_lane = 0) {
   BmLaunch a;
   a.sc = sc;
   a.input = d_input;
@@ -201,7 +202,8 @@ int launch_bm_items(const cm_bm_scalars &sc, const float *d_input, int64_t input
     rc = cm_launch_bm_ws(&a, &w, device_cus(), stream());
     rc = rc > 0 ? 0 : rc;
   } else {
-    rc = cm_launch_bm(&a, waves_per_block, unroll > 0 ? unroll : g_bm.jacobi_unroll, grid, stream());
+    rc = cm_launch_bm(&a, waves_per_block, unroll > 0 ? unroll : g_bm.jacobi_unroll,
+                      sections_per_lane > 0 ? sections_per_lane : g_bm.sections_per_lane, grid, stream());
   }
   if (rc) {
     set_error("BM kernel launch failed (%d): %s", rc, hipGetErrorString(hipGetLastError()));
@@ -343,35 +345,48 @@ int cm_bm_autotune(const cm_bm_scalars *sc, const float *d_input, int64_t input_
     return -1;
   const int saved_n = g_bm.order_n;
   g_bm.order_n = 0;
-  // trial c = 6 u + 2 w + o: waves per workgroup wpb[w], order o (natural,
-  // longest first), Jacobi unroll unr[u]
-  const int wpb[3] = {1, 2, 4}, unr[3] = {2, 4, 1};
+  // trial c < 18 (4 sections per lane): c = 6 u + 2 w + o, waves per workgroup
+  // wpb[w], order o (natural, longest first), Jacobi unroll unr[u]; trial
+  // c = 18 + 6 v + 2 u + o: spl[v] sections per lane (one block per workgroup)
+  const int wpb[3] = {1, 2, 4}, unr[3] = {2, 4, 1}, spl[2] = {2, 1};
   float best = 1e30f;
   int bi = 0;
-  for (int c = -1; c < 18; ++c) {   // c = -1: untimed warm-up
+  for (int c = -1; c < 30; ++c) {   // c = -1: untimed warm-up
     const int cc = c < 0 ? 0 : c;
-    const int u = unr[cc / 6], w = wpb[(cc % 6) / 2], o = cc % 2;
+    const bool split = cc >= 18;
+    const int t = split ? cc - 18 : cc;
+    const int o = t % 2;
+    const int uu = split ? unr[(t % 6) / 2] : unr[t / 6];
+    const int w = split ? 1 : wpb[(t % 6) / 2];
+    const int n = split ? spl[t / 6] : 4;
     float ms = 0.f;
     const int rc = launch_bm_items(*sc, d_input, input_len, d_sec, d_gamma, g_bm.trial_items.ptr, ns, d_speeds, rows,
                                    nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
-                                   w, 0, &ms, g_bm.trial_order.ptr + o * ns, u);
+                                   w, 0, &ms, g_bm.trial_order.ptr + o * ns, uu, n);
     if (rc) {
       g_bm.order_n = saved_n;
       return rc;
     }
     if (c < 0) continue;
-    out->ms[c] = ms;
+    if (split)
+      out->ms_split[t] = ms;
+    else
+      out->ms[c] = ms;
     if (ms < best) {
       best = ms;
       bi = c;
     }
   }
   g_bm.order_n = saved_n;
-  out->waves_per_block = wpb[(bi % 6) / 2];
-  out->order_lpt = bi % 2;
+  const bool bsplit = bi >= 18;
+  const int bt = bsplit ? bi - 18 : bi;
+  out->sections_per_lane = bsplit ? spl[bt / 6] : 4;
+  out->waves_per_block = bsplit ? 1 : wpb[(bt % 6) / 2];
+  out->order_lpt = bt % 2;
   out->n_trial_items = ns;
-  out->jacobi_unroll = unr[bi / 6];
+  out->jacobi_unroll = bsplit ? unr[(bt % 6) / 2] : unr[bt / 6];
   g_bm.jacobi_unroll = out->jacobi_unroll;
+  g_bm.sections_per_lane = out->sections_per_lane;
   if (out->order_lpt) return cm_set_bm_order(full.data(), n_items);
   g_bm.order_n = 0;
   return 0;
@@ -398,6 +413,15 @@ int cm_set_bm_jacobi_unroll(int unroll) {
     return -3;
   }
   g_bm.jacobi_unroll = unroll;
+  return 0;
+}
+
+int cm_set_bm_sections_per_lane(int sections_per_lane) {
+  if (sections_per_lane != 1 && sections_per_lane != 2 && sections_per_lane != 4) {
+    set_error("cm_set_bm_sections_per_lane: 1, 2 or 4 sections per lane (got %d)", sections_per_lane);
+    return -3;
+  }
+  g_bm.sections_per_lane = sections_per_lane;
   return 0;
 }
 

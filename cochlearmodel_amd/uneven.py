@@ -80,9 +80,9 @@ class UnevenBatch:
         self.stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
 
     def autotune(self, sc, trial_stride: int = 4):
-        """Pick the Jacobi unroll, waves per workgroup and launch order on a
-        sample of the blocks; the library keeps the unroll and order for the
-        following launches."""
+        """Pick the sections per lane, Jacobi unroll, waves per workgroup and
+        launch order on a sample of the blocks; the library keeps the section
+        layout, unroll and order for the following launches."""
         L = lib()
         L.cm_set_stream(self.stream)
         out = np.zeros(1, TUNING_DTYPE)
@@ -94,9 +94,10 @@ class UnevenBatch:
         return out[0]
 
     def clear_order(self):
-        """Back to the untuned launch order and Jacobi unroll."""
+        """Back to the untuned launch order, Jacobi unroll and section layout."""
         check(lib().cm_set_bm_order(None, 0), "cm_set_bm_order")
         check(lib().cm_set_bm_jacobi_unroll(2), "cm_set_bm_jacobi_unroll")
+        check(lib().cm_set_bm_sections_per_lane(4), "cm_set_bm_sections_per_lane")
 
     def run_bm(self, sc, timed: bool = True, waves_per_block: int | None = None):
         L = lib()

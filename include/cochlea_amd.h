@@ -65,6 +65,14 @@ int cm_set_bm_launch(int waves_per_block, int persistent);
  * include/cochlea.cuh:13962-13982).  Results are identical for every value;
  * cm_bm_autotune sets the fastest. */
 int cm_set_bm_jacobi_unroll(int unroll);
+/* Sections per lane (1, 2 or 4; default 4) of the following fused BM launches
+ * (the learned lockstep kernel keeps 4).  4: one wave64 per time block, 4
+ * sections per lane; 2 or 1: a block over a workgroup of 2 or 4 waves, the
+ * wave-boundary pressures and partial reductions exchanged through LDS (the
+ * reference runs one section per thread in 256-thread blocks,
+ * include/cochlea.cuh:13708-13736).  Results are identical for every value;
+ * cm_bm_autotune sets the fastest. */
+int cm_set_bm_sections_per_lane(int sections_per_lane);
 /* Solver of the BMOHCNewKernel launches (no reference counterpart): jacobi_mode
  * 0 = the reference's fixed sweeps (default), 1 = Jacobi to relative residual
  * tol_rel checked every check_every sweeps, capped at max_sweeps; warm_start 0
@@ -74,11 +82,12 @@ int cm_set_bm_solver(int jacobi_mode, float tol_rel, int check_every, int max_sw
  * integrates block h_order[w] (a permutation; results do not depend on it).
  * NULL clears it. */
 int cm_set_bm_order(const int32_t *h_order, int n);
-/* Times the launch configurations (Jacobi unroll x waves per workgroup x
- * natural or longest-first order) on every trial_stride-th block of the table
- * (taken in longest-first order) and reports the fastest; installs its unroll,
- * and the longest-first order when it wins.  Results are identical for every
- * configuration.
+/* Times the launch configurations (4 sections per lane: Jacobi unroll x waves
+ * per workgroup x natural or longest-first order; 2 and 1 sections per lane:
+ * Jacobi unroll x order) on every trial_stride-th block of the table (taken in
+ * longest-first order) and reports the fastest; installs its sections per
+ * lane and unroll, and the longest-first order when it wins.  Results are
+ * identical for every configuration.
  * h_items: the block table on the host; d_speeds receives the trial rows. */
 int cm_bm_autotune(const cm_bm_scalars *sc, const float *d_input, int64_t input_len, const float *d_sec,
                    const float *d_gamma, const cm_bm_item *h_items, int n_items, float *d_speeds, int64_t rows,

@@ -82,6 +82,11 @@ def test_bm_run_rejects_invalid_arguments_before_any_launch():
         bad.jacobi_mode, bad.warm_start, bad.check_every, bad.tol_rel, bad.max_sweeps = 1, ws, check_every, 1e-6, 1000
         assert run(bad) == -3 and "warm_start" in msg(), (ws, check_every)
     assert run(sc, n_items=0) == 0
+    # launch knobs are host state: bad values are refused, good ones kept
+    assert L.cm_set_bm_sections_per_lane(3) == -3 and "1, 2 or 4" in msg()
+    assert L.cm_set_bm_jacobi_unroll(3) == -3
+    for n in (1, 2, 4):
+        assert L.cm_set_bm_sections_per_lane(n) == 0
 
 
 def test_batched_stage_entry_points_reject_bad_sizes_before_any_launch():

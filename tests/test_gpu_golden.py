@@ -20,12 +20,20 @@ def _sha(a):
     return np.frombuffer(hashlib.sha256(np.ascontiguousarray(a).tobytes()).digest(), np.uint8)
 
 
-def test_voice_run_matches_known_answers():
-    from cochlearmodel_amd import solver, stages
-    print("case: golden C1 VOICE run, 5 blocks, BM + IHC/lambda", flush=True)
+@pytest.mark.parametrize("sections_per_lane", [4, 2, 1])
+def test_voice_run_matches_known_answers(sections_per_lane):
+    """Every block layout (4 sections per lane on one wave; 2 or 1 on a
+    workgroup of 2 or 4 waves) reproduces the committed answers bit for bit."""
+    from cochlearmodel_amd import _lib, solver, stages
+    print(f"case: golden C1 VOICE run, 5 blocks, BM + IHC/lambda, {sections_per_lane} sections per lane", flush=True)
     fx = dict(np.load(os.path.join(HERE, "golden", "oracle_c1.npz"), allow_pickle=False))
     lay = solver.voice_layout(G.FS, 5, 0.02, 0.005)
-    g = solver.run_bm_voice(fx["input"][:lay.bufP1], lay)
+    L = _lib.lib()
+    _lib.check(L.cm_set_bm_sections_per_lane(sections_per_lane))
+    try:
+        g = solver.run_bm_voice(fx["input"][:lay.bufP1], lay)
+    finally:
+        _lib.check(L.cm_set_bm_sections_per_lane(4))
     v = g["speeds"][G.ROW0:G.ROW1]
     np.testing.assert_array_equal(v[::G.STRIDE_V], fx["o1_rows"])
     st = g["stats"]

@@ -63,34 +63,40 @@ def test_launch_configurations_give_identical_results():
     from cochlearmodel_amd.uneven import UnevenBatch
     sigs, ohc, ihc = _signals(n=9, max_s=0.06)
     ub = UnevenBatch(sigs, ohc, ihc, FS)
-    _say(f"case: {ub.TB} blocks of {len(sigs)} uneven signals, Jacobi unroll 1/2/4 x waves 1/2/4 x "
-         "natural/longest-first order, fixed sweeps and both tolerance modes")
+    _say(f"case: {ub.TB} blocks of {len(sigs)} uneven signals, sections per lane 4 (waves 1/2/4) / 2 / 1 x "
+         "Jacobi unroll 1/2/4 x natural/longest-first order, fixed sweeps and both tolerance modes")
     ref = {}
-    for unroll in (2, 1, 4):
-        _lib.check(_lib.lib().cm_set_bm_jacobi_unroll(unroll))
-        for order in (False, True):
-            if order:
-                lens = ub.items["mesi"] - ub.items["start"]
-                ordr = np.argsort(-lens, kind="stable").astype(np.int32)
-                _lib.check(_lib.lib().cm_set_bm_order(_lib.ptr(ordr), len(ordr)))
-            else:
-                _lib.check(_lib.lib().cm_set_bm_order(None, 0))
-            for wpb in (1, 2, 4):
-                for ws, tol in ((0, False), (0, True), (1, True)):
-                    sc = solver.default_scalars(FS)
-                    if tol:
-                        sc.jacobi_mode, sc.tol_rel, sc.check_every, sc.max_sweeps, sc.warm_start = 1, 1e-6, 5, 1000, ws
-                    ub.d_speeds.zero_()
-                    ub.run_bm(sc, timed=False, waves_per_block=wpb)
-                    torch.cuda.synchronize()
-                    out = (ub.d_speeds.cpu().numpy().copy(), ub.d_stats.cpu().numpy().copy())
-                    key = (ws, tol)
-                    if key not in ref:
-                        ref[key] = out
-                    else:
-                        np.testing.assert_array_equal(out[0], ref[key][0], err_msg=f"unroll {unroll} wpb {wpb}")
-                        np.testing.assert_array_equal(out[1], ref[key][1], err_msg=f"unroll {unroll} wpb {wpb}")
+    for spl in (4, 2, 1):
+        _lib.check(_lib.lib().cm_set_bm_sections_per_lane(spl))
+        for unroll in (2, 1, 4):
+            _lib.check(_lib.lib().cm_set_bm_jacobi_unroll(unroll))
+            for order in (False, True):
+                if order:
+                    lens = ub.items["mesi"] - ub.items["start"]
+                    ordr = np.argsort(-lens, kind="stable").astype(np.int32)
+                    _lib.check(_lib.lib().cm_set_bm_order(_lib.ptr(ordr), len(ordr)))
+                else:
+                    _lib.check(_lib.lib().cm_set_bm_order(None, 0))
+                for wpb in ((1, 2, 4) if spl == 4 else (1,)):
+                    for ws, tol in ((0, False), (0, True), (1, True)):
+                        sc = solver.default_scalars(FS)
+                        if tol:
+                            sc.jacobi_mode, sc.tol_rel, sc.check_every, sc.max_sweeps, sc.warm_start = \
+                                1, 1e-6, 5, 1000, ws
+                        ub.d_speeds.zero_()
+                        ub.run_bm(sc, timed=False, waves_per_block=wpb)
+                        torch.cuda.synchronize()
+                        out = (ub.d_speeds.cpu().numpy().copy(), ub.d_stats.cpu().numpy().copy())
+                        key = (ws, tol)
+                        what = f"spl {spl} unroll {unroll} wpb {wpb} order {order}"
+                        if key not in ref:
+                            ref[key] = out
+                        else:
+                            np.testing.assert_array_equal(out[0], ref[key][0], err_msg=what)
+                            np.testing.assert_array_equal(out[1], ref[key][1], err_msg=what)
     ub.clear_order()
+    with pytest.raises(_lib.CochleaError):
+        _lib.check(_lib.lib().cm_set_bm_sections_per_lane(3))
     # unroll 4 runs the reference's 20 sweeps as five trips and 2 as one pair;
     # tolerance checks every 5 sweeps take one trip and a single sweep
     with pytest.raises(_lib.CochleaError):
