@@ -11,6 +11,9 @@
  *   - IHC + lambda            src/cochlea.cu:2712-2790, 2980-2990, 3200-3240, 3365-3408, 3431-3506
  *   - JND Fisher sums         src/cochlea.cu:3741-3944
  *   - profile input synthesis src/cochlea.cu:4070-4103
+ * and the learned predictor-pass guess this build adds to kernel 17 (no
+ * counterpart in the reference): the same features, network, smoothing and
+ * residual gate as the GPU's lockstep kernel, with the network on the host.
  *
  * Arithmetic contract shared with the GPU kernels (not with their code):
  *   * FP32 state, every operation IEEE-rounded, no FMA contraction
@@ -29,7 +32,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
@@ -233,6 +238,106 @@ Decision<T> step_control(const T *aref, const T *na, const T *vref, const T *nv,
   return d;
 }
 
+// ------------------------------------------------ learned warm start ----
+// The predictor-pass guess network (no counterpart in the reference): x (769)
+// -> GELU(x W1 + b1) (512) -> GELU(. W2 + b2) (512) -> . W3 + b3 (256), with
+// every GEMM operand rounded to bfloat16 as the GPU rounds it, float
+// accumulation and float bias adds.  The weights are kept transposed (one
+// contiguous row per output) for the dot products.
+constexpr int WS_IN = 769, WS_H = 512, WS_OUT = NS;
+struct WsNet {
+  std::vector<uint16_t> w1t, w2t, w3t;   // bfloat16 bit patterns, [out][in]
+  std::vector<float> b1, b2, b3;
+  float c_out = 0.f;
+  bool loaded = false;
+};
+WsNet g_net;
+
+float bf16_round(float x) {   // nearest even; NaN stays a (quiet) NaN
+  uint32_t u;
+  std::memcpy(&u, &x, 4);
+  if ((u & 0x7fffffffu) > 0x7f800000u) {
+    u = (u | 0x00400000u) & 0xffff0000u;
+  } else {
+    u += 0x7fffu + ((u >> 16) & 1u);
+    u &= 0xffff0000u;
+  }
+  float r;
+  std::memcpy(&r, &u, 4);
+  return r;
+}
+
+// GELU, tanh form written as x sigmoid(2u), u = sqrt(2/pi) (x + 0.044715 x^3):
+// x / (1 + 2^z), z = -2 u log2(e).
+float ws_gelu(float x) {
+  const float t = x * x;
+  const float z = (x * std::fma(0.044715f, t, 1.0f)) * -2.302208198f;
+  return x / (1.0f + std::exp2(z));
+}
+
+// out[j] = sum_k x[k] w[j][k] + b[j] over K inputs: 16 interleaved float
+// partial sums per output (two 8-wide vectors, multiply then add), combined
+// pairwise, four outputs per pass over x (a fixed order, so runs are
+// reproducible).  The weights are bfloat16 bit patterns (the network's
+// precision), widened to float as they are read: half the bytes of float
+// weights, which is what bounds this loop.
+typedef float ws_v8 __attribute__((vector_size(32)));
+typedef uint32_t ws_u8 __attribute__((vector_size(32)));
+typedef uint16_t ws_h8 __attribute__((vector_size(16)));
+inline ws_v8 ws_ld8(const float *p) {
+  ws_v8 v;
+  std::memcpy(&v, p, sizeof(v));
+  return v;
+}
+inline ws_v8 ws_ldb8(const uint16_t *p) {
+  ws_h8 h;
+  std::memcpy(&h, p, sizeof(h));
+  const ws_u8 u = __builtin_convertvector(h, ws_u8) << 16;
+  ws_v8 v;
+  std::memcpy(&v, &u, sizeof(v));
+  return v;
+}
+inline float ws_b2f(uint16_t h) {
+  const uint32_t u = (uint32_t)h << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+void ws_gemv(const float *x, const uint16_t *wt, int K, int N, const float *b, float *out) {
+  const int K16 = K & ~15;
+  for (int j = 0; j < N; j += 4) {
+    const uint16_t *w0 = wt + (size_t)j * K, *w1 = w0 + K, *w2 = w1 + K, *w3 = w2 + K;
+    ws_v8 a0l = {}, a0h = {}, a1l = {}, a1h = {}, a2l = {}, a2h = {}, a3l = {}, a3h = {};
+    for (int k = 0; k < K16; k += 16) {
+      const ws_v8 xl = ws_ld8(x + k), xh = ws_ld8(x + k + 8);
+      a0l += xl * ws_ldb8(w0 + k); a0h += xh * ws_ldb8(w0 + k + 8);
+      a1l += xl * ws_ldb8(w1 + k); a1h += xh * ws_ldb8(w1 + k + 8);
+      a2l += xl * ws_ldb8(w2 + k); a2h += xh * ws_ldb8(w2 + k + 8);
+      a3l += xl * ws_ldb8(w3 + k); a3h += xh * ws_ldb8(w3 + k + 8);
+    }
+    const ws_v8 *acc[4][2] = {{&a0l, &a0h}, {&a1l, &a1h}, {&a2l, &a2h}, {&a3l, &a3h}};
+    const uint16_t *wr[4] = {w0, w1, w2, w3};
+    for (int r = 0; r < 4; ++r) {
+      float a[16];
+      std::memcpy(a, acc[r][0], 32);
+      std::memcpy(a + 8, acc[r][1], 32);
+      for (int k = K16; k < K; ++k) a[k & 15] = a[k & 15] + x[k] * ws_b2f(wr[r][k]);
+      for (int h = 8; h >= 1; h >>= 1)
+        for (int u = 0; u < h; ++u) a[u] = a[u] + a[u + h];
+      out[j + r] = a[0] + b[j + r];
+    }
+  }
+}
+
+void ws_forward(const float *x, float *y) {
+  float h1[WS_H], h2[WS_H];
+  ws_gemv(x, g_net.w1t.data(), WS_IN, WS_H, g_net.b1.data(), h1);
+  for (int j = 0; j < WS_H; ++j) h1[j] = bf16_round(ws_gelu(h1[j]));
+  ws_gemv(h1, g_net.w2t.data(), WS_H, WS_H, g_net.b2.data(), h2);
+  for (int j = 0; j < WS_H; ++j) h2[j] = bf16_round(ws_gelu(h2[j]));
+  ws_gemv(h2, g_net.w3t.data(), WS_H, WS_OUT, g_net.b3.data(), y);
+}
+
 // ------------------------------------------------------ kernel #17 -------
 // Optional per-pass record of one item's run (orc_bm_trace_f32).
 struct Trace {
@@ -402,6 +507,47 @@ void bm_item(const orc_bm_scalars &sc, const float *sec, const float *gamma_set,
           std::memcpy(pold, p, sizeof(T) * NS);
           if (rre < rr0) std::memcpy(p, pext, sizeof(T) * NS);
           dt_prev = dt;
+        } else if (sc.warm_start == 2 && out_loop == 0) {
+          // learned guess p_lin + rms(r_lin) c_out S(y), S = [1,2,1]/4 with the
+          // edges replicated, y = the network of the features (d, v, r_lin,
+          // each over its RMS, and s over the RMS of the previous pressure);
+          // the smallest residual of the previous, extrapolated and learned
+          // guesses is kept (ties: extrapolated over learned, previous over
+          // extrapolated)
+          const T rho = dt / dt_prev;
+          for (int i = 0; i < NS; ++i) pext[i] = p[i] + rho * (p[i] - pold[i]);
+          const T rr0 = resid2(p), rre = resid2(pext);
+          std::memcpy(pold, p, sizeof(T) * NS);
+          dt_prev = dt;
+          auto rms = [&](const T *v) {
+            T q[NS];
+            for (int i = 0; i < NS; ++i) q[i] = v[i] * v[i];
+            return (T)std::sqrt(tree_sum(q) * ((T)1 / (T)256)) + (T)1e-30f;
+          };
+          const T rr_scale = (T)std::sqrt(rre * ((T)1 / (T)256)) + (T)1e-30f;
+          const T inv_d = (T)1 / rms(nd), inv_v = (T)1 / rms(nv), inv_r = (T)1 / rr_scale, inv_p = (T)1 / rms(pold);
+          float x[WS_IN], y[WS_OUT];
+          for (int i = 0; i < NS; ++i) {
+            T pm1 = i > 0 ? pext[i - 1] : (T)0;
+            T pp1 = i < NS - 1 ? pext[i + 1] : (T)0;
+            T r = Y[i] - ((low[i] * pm1 + pp1) + mf[i] * pext[i]);
+            x[i] = bf16_round((float)(nd[i] * inv_d));
+            x[NS + i] = bf16_round((float)(nv[i] * inv_v));
+            x[2 * NS + i] = bf16_round((float)(r * inv_r));
+          }
+          x[3 * NS] = bf16_round((float)(s * inv_p));
+          ws_forward(x, y);
+          const T sc_out = rr_scale * (T)g_net.c_out;
+          T *ph = pn;
+          for (int i = 0; i < NS; ++i) {
+            const T yl = (T)y[i > 0 ? i - 1 : 0], yc = (T)y[i], yr = (T)y[i < NS - 1 ? i + 1 : NS - 1];
+            ph[i] = pext[i] + sc_out * (((T)0.25f * yl + (T)0.5f * yc) + (T)0.25f * yr);
+          }
+          const T rrh = resid2(ph);
+          if (rrh < rre && rrh < rr0)
+            std::memcpy(p, ph, sizeof(T) * NS);
+          else if (rre < rr0)
+            std::memcpy(p, pext, sizeof(T) * NS);
         }
         J = 0;
         for (;;) {
@@ -506,6 +652,7 @@ template <typename T, typename C = T>
 int bm_run(const orc_bm_scalars *sc, const float *sec, const float *gamma, const orc_bm_item *items, int n_items,
            const float *input, int64_t input_len, const Outs<T> &o, int n_threads) {
   if (!sc || !sec || !gamma || !items || !input || n_items < 0) return -1;
+  if (sc->warm_start == 2 && (!g_net.loaded || !std::is_same<T, float>::value || sc->jacobi_mode != 1)) return -2;
 #ifdef _OPENMP
   if (n_threads > 0) omp_set_num_threads(n_threads);
 #pragma omp parallel for schedule(dynamic, 1)
@@ -670,6 +817,40 @@ int orc_bm_run_f64_clock32(const orc_bm_scalars *sc, const float *sec, const flo
   Outs<double> o{speeds, rows, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
                  item_steps, item_passes, item_sweeps};
   return bm_run<double, float>(sc, sec, gamma, items, n_items, input, input_len, o, n_threads);
+}
+
+int orc_ws_forward(const float *x, int n_rows, float *y) {
+  if (!g_net.loaded || !x || !y || n_rows < 0) return -1;
+  for (int r = 0; r < n_rows; ++r) {
+    float xb[WS_IN];
+    for (int k = 0; k < WS_IN; ++k) xb[k] = bf16_round(x[(size_t)r * WS_IN + k]);
+    ws_forward(xb, y + (size_t)r * WS_OUT);
+  }
+  return 0;
+}
+
+int orc_ws_set_net(const float *w1, const float *b1, const float *w2, const float *b2, const float *w3,
+                   const float *b3, float c_out) {
+  if (!w1 || !b1 || !w2 || !b2 || !w3 || !b3) return -1;
+  auto tr = [](const float *w, int K, int N, std::vector<uint16_t> &out) {   // [K][N] -> rounded [N][K]
+    out.assign((size_t)K * N, 0);
+    for (int k = 0; k < K; ++k)
+      for (int j = 0; j < N; ++j) {
+        const float r = bf16_round(w[(size_t)k * N + j]);
+        uint32_t u;
+        std::memcpy(&u, &r, 4);
+        out[(size_t)j * K + k] = (uint16_t)(u >> 16);
+      }
+  };
+  tr(w1, WS_IN, WS_H, g_net.w1t);
+  tr(w2, WS_H, WS_H, g_net.w2t);
+  tr(w3, WS_H, WS_OUT, g_net.w3t);
+  g_net.b1.assign(b1, b1 + WS_H);
+  g_net.b2.assign(b2, b2 + WS_H);
+  g_net.b3.assign(b3, b3 + WS_OUT);
+  g_net.c_out = c_out;
+  g_net.loaded = true;
+  return 0;
 }
 
 int orc_jacobi(const float *Y, const float *rsm, float *p, int sweeps) {

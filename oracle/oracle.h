@@ -50,7 +50,8 @@ typedef struct {
   float tol_rel;        /* tolerance mode: stop when |Y - A p| <= tol_rel |Y| */
   int32_t warm_start;   /* tolerance mode, predictor pass: 0 = previous pressure; 1 = also try the
                            linear extrapolation p + (dt/dt_prev)(p - p_prev) and keep the guess
-                           with the smaller residual (ties keep the previous pressure) */
+                           with the smaller residual (ties keep the previous pressure); 2 = also
+                           the learned guess (orc_ws_set_net; FP32 run only) */
 } orc_bm_scalars;
 
 /* One work item = one reference "time block" (src/cochlea.cu:1148-1176). */
@@ -124,6 +125,17 @@ int orc_bm_trace_f32(const orc_bm_scalars *sc, const float *sec, const float *ga
 int orc_bm_run_f64_clock32(const orc_bm_scalars *sc, const float *sec, const float *gamma, const orc_bm_item *items,
                            int n_items, const float *input, int64_t input_len, double *speeds, int64_t rows,
                            int64_t *item_steps, int64_t *item_passes, int64_t *item_sweeps, int n_threads);
+
+/* Network of the learned predictor-pass guess (warm_start = 2 in the FP32
+ * run; no counterpart in the reference): W1 769 x 512, W2 512 x 512, W3 512 x
+ * 256 row-major float32 (rounded to bfloat16 here), biases 512, 512, 256,
+ * output scale c_out.  Copied; process-wide. */
+int orc_ws_set_net(const float *w1, const float *b1, const float *w2, const float *b2, const float *w3,
+                   const float *b3, float c_out);
+
+/* The network on n_rows feature rows (769 floats each, rounded to bfloat16
+ * here) -> 256 outputs per row. */
+int orc_ws_forward(const float *x, int n_rows, float *y);
 
 /* Jacobi sweeps of the kernel convention (Q7) on one right-hand side. */
 int orc_jacobi(const float *Y, const float *rsm, float *p, int sweeps);

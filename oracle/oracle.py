@@ -100,6 +100,25 @@ def section_params() -> np.ndarray:
     return out
 
 
+def set_ws_net(params, c_out: float):
+    """Network of the learned predictor-pass guess (warm_start = 2):
+    params = [W1 (769 x 512), b1, W2 (512 x 512), b2, W3 (512 x 256), b3]."""
+    shapes = [(769, 512), (512,), (512, 512), (512,), (512, 256), (256,)]
+    arr = [np.ascontiguousarray(p, np.float32) for p in params]
+    assert [a.shape for a in arr] == shapes, [a.shape for a in arr]
+    rc = lib().orc_ws_set_net(*[_p(a) for a in arr], C.c_float(c_out))
+    assert rc == 0, rc
+
+
+def ws_forward(x: np.ndarray) -> np.ndarray:
+    """The network set by set_ws_net on rows of 769 features."""
+    x = np.ascontiguousarray(np.atleast_2d(x), np.float32)
+    y = np.zeros((x.shape[0], 256), np.float32)
+    rc = lib().orc_ws_forward(_p(x), C.c_int(x.shape[0]), _p(y))
+    assert rc == 0, rc
+    return y
+
+
 def block_decomposition(time_blocks: int, bufP1: int, overlap: int, decouple_filter: int):
     a = [np.zeros(time_blocks, np.int32) for _ in range(4)]
     rc = lib().orc_block_decomposition(time_blocks, bufP1, overlap, decouple_filter, *[_p(x) for x in a])

@@ -224,3 +224,39 @@ def test_lockstep_grouping_does_not_change_results():
         _lib.check(L.cm_set_bm_order(None, 0))
     np.testing.assert_array_equal(v1, v2)
     np.testing.assert_array_equal(s1, s2)
+
+
+def test_learned_solver_matches_its_cpu_port(oracle):
+    """The whole learned run against the oracle's port of it (the same
+    features, network, smoothing and residual gate on the CPU): the CPU network
+    sums in its own order, so guesses differ in the last bits and the two runs
+    stop their Jacobi iterations at different iterates within the same
+    tolerance; committed steps, sweeps per step and velocity agree to that
+    level."""
+    _cuda()
+    from cochlearmodel_amd import _lib, warmstart
+    params, c_out, _ = warmstart.load_model()
+    warmstart.upload(params, c_out)
+    oracle.set_ws_net(params, c_out)
+    _lib.check(_lib.lib().cm_ws_set_every(1))
+    _say("case: trained network, 8 held-out tones x 100 ms, GPU learned run against the oracle port")
+    tb = _batch(n=8, dur=0.1, seed=warmstart.EVAL_SEED + 2000)
+    v, st = _run(tb, warmstart.tol_scalars(1e-6, 2))
+    sel = tb.sample_blocks(8)
+    it = np.zeros(len(sel), oracle.ITEM_DTYPE)
+    for f in ("start", "mosi", "mesi", "write_start", "tol", "m1tol", "set"):
+        it[f] = tb.items[f][sel]
+    so = oracle.default_scalars(FS)
+    so.jacobi_mode, so.tol_rel, so.check_every, so.max_sweeps, so.warm_start = 1, 1e-6, 4, 1000, 2
+    o = oracle.bm_run(so, oracle.section_params(), np.full(256, 0.5, np.float32), it, tb.d_input.cpu().numpy(),
+                      tb.bufP1, "f32", threads=8)
+    rows = tb.block_rows(sel)
+    d = float(np.linalg.norm(v[rows] - o["speeds"][rows]) / np.linalg.norm(o["speeds"][rows]))
+    g_steps, g_sweeps = int(st[sel, 0].sum()), int(st[sel, 2].sum())
+    c_steps, c_sweeps = int(o["steps"].sum()), int(o["sweeps"].sum())
+    _say(f"steps {g_steps} vs {c_steps}, sweeps/step {g_sweeps / g_steps:.3f} vs {c_sweeps / c_steps:.3f}, "
+         f"rel L2 {d:.2e}")
+    # measured: equal steps, sweeps per step 0.3% apart, rel-L2 1.5e-4
+    assert abs(g_steps - c_steps) <= 0.01 * c_steps
+    assert abs(g_sweeps / g_steps - c_sweeps / c_steps) <= 0.02 * (c_sweeps / c_steps)
+    assert d <= 3e-4
