@@ -832,7 +832,8 @@ def c2_main(args):
         "config": {"workload": "256 tones (32 freq 250-8000 Hz x 8 levels 20-90 dB), 200 ms at 44.1 kHz, "
                                "256 sections, Decouple_Filter=10, 2560 time blocks, input synthesis + BM + "
                                "IHC/lambda per step; BM solver: " + desc[args.variant],
-                   "variant": args.variant, "signals": work.n, "time_blocks": work.TB, "rows": work.rows},
+                   "variant": args.variant, "signals": work.n, "time_blocks": work.TB, "rows": work.rows,
+                   "sections_per_lane": 4 if args.variant == "learned_tol" else args.sections_per_lane},
         "jacobi_iters_per_step": sweeps / steps, "passes_per_step": passes / steps,
         "committed_steps": steps, "signals_per_s": work.n / (ms_per_step * 1e-3),
         "bm_ms": bm_avg, "ihc_lambda_ms": float(np.mean(ihc_ms)), "input_ms": float(np.mean(in_ms)),
@@ -939,6 +940,8 @@ def main():
     ap.add_argument("--c3-signals", type=int, default=1024)
     ap.add_argument("--c4-intervals", type=int, default=8, help="calculated intervals checked on the CPU")
     ap.add_argument("--no-graph", action="store_true", help="c4: launch the post-BM stages eagerly")
+    ap.add_argument("--sections-per-lane", type=int, default=4, choices=(4, 2, 1),
+                    help="block layout of the fused BM kernels (the learned kernel keeps 4)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -956,6 +959,8 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    from cochlearmodel_amd import _lib
+    _lib.check(_lib.lib().cm_set_bm_sections_per_lane(args.sections_per_lane), "cm_set_bm_sections_per_lane")
     if args.workload == "c1":
         return c1_main(args)
     if args.workload == "c3":
