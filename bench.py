@@ -79,7 +79,7 @@ MLP_FLOP_PER_STEP = 2.0 * (769 * 512 + 512 * 512 + 512 * 256)   # forward, one b
 # WRITE_SIZE (KiB per dispatch), scaled by what the counters read for a 1 GiB
 # device copy (FETCH x 2.0, WRITE x 1.0); profiles/pmc_traffic_c2.md
 MEASURED_HBM_BYTES = {
-    "learned_tol": (47479.375 * 2.0 + 2265904.0) * 1024,
+    "learned_tol": (12274.4375 * 2.0 + 2258200.0) * 1024,
     "reference_20_2": (6085.375 * 2.0 + 2258260.0) * 1024,
     "ihc_lambda": (1150991.3125 * 2.0 + 4515840.0 + 2288920.5 * 2.0 + 6773760.0) * 1024,
 }
@@ -88,7 +88,7 @@ PEAK_BF16_TFLOPS = 2500.0    # MI355X dense BF16 MFMA
 # profiles/pmc_valu_c2_reference.md, profiles/pmc_mix_c2_learned.md) and the issue
 # capacity they are measured against: a wave64 FP32 VALU instruction holds a SIMD
 # for 4 cycles, 1024 SIMDs at 2.4 GHz
-MEASURED_VALU_INSTS = {"reference_20_2": 1.0995e11, "learned_tol": 1.6547e11,
+MEASURED_VALU_INSTS = {"reference_20_2": 1.0995e11, "learned_tol": 1.64671514484e11,
                        "ihc_lambda": 1.40262e9 + 2.68728e9}   # ac_ds_kernel + dc_lambda_kernel
 # VALU issue capacity: the rate bm_kernel's own instruction mix reaches at full
 # occupancy (SQ_INSTS_VALU / kernel time on the JND sweep's 48300 blocks,
@@ -107,16 +107,16 @@ PEAK_FP32_UNPACKED_TFLOPS = PEAK_FP32_TFLOPS / 2
 # longest block first; profiles/pmc_traffic_c5_learned.md, profiles/pmc_mix_c5_learned.md)
 C5_MEASURED = {
     "tones_per_launch": 2048,
-    "hbm_bytes": {"learned_tol": (920172.375 * 2.0 + 18125642.5) * 1024,
+    "hbm_bytes": {"learned_tol": (1438213185.0 * 2.0 + 1957872930.34375) * 1024,
                   "ihc_lambda": (9206258.75 * 2.0 + 36126720.0 + 18361109.0625 * 2.0 + 54190080.0) * 1024},
-    "valu_insts": {"learned_tol": 1.310250600478e12, "plain_tol": 1.269936053668e12,
+    "valu_insts": {"learned_tol": 1.284127165574e12, "plain_tol": 1.269936053668e12,
                    "ihc_lambda": 1.1221170816e10 + 2.1498514912e10},
-    "l2_read_bytes": {"learned_tol": 5.23537354631e11 * 128},
-    "mfma_insts": {"learned_tol": 6.360480032e10},
+    "l2_read_bytes": {"learned_tol": 4.15991651896e11 * 128},
+    "mfma_insts": {"learned_tol": 4.769789056e10},
 }
 # L1<-L2 reads of the learned kernel: TCP_TCC_READ_REQ_sum x 128 B (a 1 GiB copy
 # reads 8.389e6 requests); profiles/pmc_mix_c2_learned.md
-MEASURED_L2_READ_BYTES = {"learned_tol": 6.6959e10 * 128}
+MEASURED_L2_READ_BYTES = {"learned_tol": 6.5295786908e10 * 128}
 
 
 def load_warm_start_model():
@@ -694,6 +694,11 @@ def c5_main(args):
         out["roofline"]["traffic"] = M["hbm_bytes"][args.variant]
         out["roofline"]["traffic_note"] = ("HBM bytes of one launch (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, calibrated "
                                            "on a 1 GiB copy); algorithmic: input 72 MB + velocity 18.5 GB")
+        if args.variant == "learned_tol":
+            out["roofline"]["traffic_note"] += (
+                "; the learned kernel's groups of 16 (4 waves per SIMD, 128 VGPRs) spill 156 B per lane, whose "
+                "stores and reloads overflow the 4 MB L2 of each XCD: almost all of the traffic, and still the "
+                "fastest form of the launch (spill-free groups of 12 move 20 GB but take 6% longer)")
     if args.variant in M["valu_insts"]:
         out["roofline"]["valu_issue"] = {
             "insts_per_launch": M["valu_insts"][args.variant],
@@ -712,14 +717,14 @@ def c5_main(args):
                                "flop_per_launch": mlp_flop / args.steps}
         if "learned_tol" in M["mfma_insts"]:
             out["mlp_roofline"]["executed_flop_per_launch"] = M["mfma_insts"]["learned_tol"] * 16 * 16 * 32 * 2
-            out["mlp_roofline"]["executed_note"] = ("v_mfma_f32_16x16x32_bf16 count x 16384: 12 block rows padded to "
-                                                    "16, 769 inputs padded to 800")
+            out["mlp_roofline"]["executed_note"] = ("v_mfma_f32_16x16x32_bf16 count x 16384: groups of 16 blocks fill "
+                                                    "the 16-row tile, 769 inputs padded to 800")
             out["roofline"]["l2_read"] = {"bytes_per_launch": M["l2_read_bytes"]["learned_tol"],
                                           "achieved": M["l2_read_bytes"]["learned_tol"] / per_launch_s / 1e12,
                                           "unit": "TB/s",
                                           "note": "L1<-L2 reads (TCP_TCC_READ_REQ x 128 B): the network's 1.606 MB "
-                                                  "of BF16 weight fragments streamed once per lockstep round of 12 "
-                                                  "blocks"}
+                                                  "of BF16 weight fragments streamed once per lockstep round of 16 "
+                                                  "blocks, and spill reloads"}
             # the two limits the lockstep round alternates between, back to back:
             # VALU issue (solver) and L2->L1 fill (weight stream)
             valu_s = M["valu_insts"]["learned_tol"] / VALU_ISSUE_PER_S

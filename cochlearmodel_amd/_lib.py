@@ -157,6 +157,7 @@ _SIGS = {
     "cm_adam": (C.c_int, [vp, vp, vp, vp, vp, i64, f32, f32, f32, f32, C.c_int, f32]),
     "cm_ws_load": (C.c_int, [vp, vp, vp, vp, vp, vp, f32]),
     "cm_ws_set_every": (C.c_int, [C.c_int]),
+    "cm_ws_set_group": (C.c_int, [C.c_int]),
     "cm_ws_set_capture": (C.c_int, [vp, i64, C.c_int, C.c_int]),
     "cm_ws_load_file": (C.c_int, [C.c_char_p]),
     "cm_ws_targets": (C.c_int, [vp, vp, i64, f32, vp]),

@@ -326,6 +326,10 @@ struct ItemStateN {
 using ItemConst = ItemConstN<4>;
 using ItemState = ItemStateN<4>;
 
+// The functions below take the per-lane section constants as K: ItemConstN<N>
+// (every row in registers) or a type with the same members whose rarely read
+// rows are pointers into LDS (the learned kernel's WsConst).
+
 // a_i = l_i p_{i-1} + p_{i+1} of slot j (include/cochlea.cuh:13962-13982).  l_i
 // is 0 or 1, so l_i p_{i-1} is exact and fma(l_i, p_{i-1}, p_{i+1}) rounds as
 // the separate multiply and add do; the fma is used where both operands are
@@ -340,8 +344,8 @@ __device__ __forceinline__ float coupling(int j, float low, float pl, float pr) 
 
 // One Jacobi sweep of the kernel convention, p_i <- (Y_i - a_i) / m_i with the
 // reciprocal diagonal.
-template <int N>
-__device__ __forceinline__ void jacobi_sweep(Lay<N> &L, const float (&Y)[N], const ItemConstN<N> &k,
+template <int N, class K>
+__device__ __forceinline__ void jacobi_sweep(Lay<N> &L, const float (&Y)[N], const K &k,
                                              const float (&p)[N], float (&q)[N]) {
   float left, right;
   L.nbrs(p[N - 1], p[0], left, right);
@@ -356,8 +360,8 @@ __device__ __forceinline__ void jacobi_sweep(Lay<N> &L, const float (&Y)[N], con
 // sweeps come in pairs so the two iterates alternate between register sets
 // instead of being copied back each sweep; U = 4 runs two pairs per loop trip.
 // Every U performs the same sweeps in the same order, so results are identical.
-template <int U = 2, int N>
-__device__ __forceinline__ void jacobi_sweeps(Lay<N> &L, const float (&Y)[N], const ItemConstN<N> &k, float (&p)[N],
+template <int U = 2, int N, class K>
+__device__ __forceinline__ void jacobi_sweeps(Lay<N> &L, const float (&Y)[N], const K &k, float (&p)[N],
                                               int J) {
   static_assert(U == 1 || U == 2 || U == 4, "unroll 1, 2 or 4");
   J = __builtin_amdgcn_readfirstlane(J);
@@ -390,8 +394,8 @@ __device__ __forceinline__ void jacobi_sweeps(Lay<N> &L, const float (&Y)[N], co
 
 // Residual Y - A p of the kernel convention, per lane, and its squared 2-norm
 // with the fixed tree.
-template <int N>
-__device__ __forceinline__ void resid_vec(Lay<N> &L, const float (&Y)[N], const ItemConstN<N> &k, const float (&p)[N],
+template <int N, class K>
+__device__ __forceinline__ void resid_vec(Lay<N> &L, const float (&Y)[N], const K &k, const float (&p)[N],
                                           float (&r)[N]) {
   float left, right;
   L.nbrs(p[N - 1], p[0], left, right);
@@ -401,8 +405,8 @@ __device__ __forceinline__ void resid_vec(Lay<N> &L, const float (&Y)[N], const 
     r[j] = Y[j] - (coupling<N>(j, k.low, pl, pr) + k.mf[j] * p[j]);
   }
 }
-template <int N>
-__device__ __forceinline__ float resid2(Lay<N> &L, const float (&Y)[N], const ItemConstN<N> &k, const float (&p)[N]) {
+template <int N, class K>
+__device__ __forceinline__ float resid2(Lay<N> &L, const float (&Y)[N], const K &k, const float (&p)[N]) {
   float r[N];
   resid_vec(L, Y, k, p, r);
   UNROLL for (int j = 0; j < N; ++j) r[j] = r[j] * r[j];
@@ -499,8 +503,8 @@ __device__ __forceinline__ void pass_begin(ItemStateN<N> &s, bool predictor) {
 
 // TM/OHC block (passes < 2, :13839-13898), G (:13905-13909) and the right-hand
 // side with its boundaries (:13925-13954).  Returns the forcing s(t).
-template <int N>
-__device__ __forceinline__ float pass_rhs(const ItemConstN<N> &k, const ItemScal &c, ItemStateN<N> &s, bool aux_pass,
+template <int N, class K>
+__device__ __forceinline__ float pass_rhs(const K &k, const ItemScal &c, ItemStateN<N> &s, bool aux_pass,
                                           const Lay<N> &L, float (&od)[N], float (&G)[N], float (&Y)[N]) {
   UNROLL for (int j = 0; j < N; ++j) {
     od[j] = s.nxi[j] - s.nd[j];
@@ -536,8 +540,8 @@ __device__ __forceinline__ float pass_rhs(const ItemConstN<N> &k, const ItemScal
 // Jacobi until |Y - A p|_2 <= tol |Y|_2 (checked every check_every sweeps,
 // capped at max_sweeps); returns the sweeps done.  rr is |Y - A p|_2^2 of the
 // incoming p (the guess selection has already computed it).
-template <int U = 2, int N>
-__device__ __forceinline__ int jacobi_tol(Lay<N> &L, const cm_bm_scalars &sc, const ItemConstN<N> &k,
+template <int U = 2, int N, class K>
+__device__ __forceinline__ int jacobi_tol(Lay<N> &L, const cm_bm_scalars &sc, const K &k,
                                           const float (&Y)[N], float yy, float (&p)[N], float rr) {
   const float tau2 = sc.tol_rel * sc.tol_rel;
   int J = 0;
@@ -552,8 +556,8 @@ __device__ __forceinline__ int jacobi_tol(Lay<N> &L, const cm_bm_scalars &sc, co
 }
 
 // Acceleration, oval-window acceleration and psi derivative, :13989-14013.
-template <int N>
-__device__ __forceinline__ void pass_finish(Lay<N> &L, const ItemConstN<N> &k, const ItemScal &c, ItemStateN<N> &s,
+template <int N, class K>
+__device__ __forceinline__ void pass_finish(Lay<N> &L, const K &k, const ItemScal &c, ItemStateN<N> &s,
                                             const float (&od)[N], const float (&G)[N], float sf) {
   UNROLL for (int j = 0; j < N; ++j) s.na[j] = (s.p[j] + G[j]) * k.rmass[j];
   {
@@ -677,8 +681,8 @@ __device__ __forceinline__ void item_final(const BmLaunch &a, const ItemStateN<N
 // pold[64].x, one copy per wave).  Returns the squared residual of the plain
 // guess; *rre gets the extrapolation's, and p holds the extrapolation on
 // return when keep_lin, else the choice.
-template <int N>
-__device__ __forceinline__ float extrapolate_guess(Lay<N> &L, const ItemConstN<N> &k, const float (&Y)[N], float dt,
+template <int N, class K>
+__device__ __forceinline__ float extrapolate_guess(Lay<N> &L, const K &k, const float (&Y)[N], float dt,
                                                    float4 *pold, float (&p)[N], float &rre, bool keep_lin) {
   float *dt_prev = reinterpret_cast<float *>(pold + 64);
   float *mine = reinterpret_cast<float *>(pold + L.lane);

@@ -27,7 +27,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(3)))
 // A workgroup of G waves integrates G time blocks in lockstep, one step per
 // round.  At each step's predictor pass every wave writes its features to
 // LDS and the G waves evaluate the 769 -> 512 -> 512 -> 256 GELU MLP together
-// on MFMA (v_mfma_f32_16x16x16_bf16, rows = blocks, padded to 16), with the
+// on MFMA (v_mfma_f32_16x16x32_bf16, rows = blocks, padded to 16), with the
 // BF16 weights streamed from L2 in MFMA fragment order.  Each wave then takes
 // the plain, extrapolated or learned guess with the smallest residual and
 // iterates Jacobi to tolerance as in the fused kernel.
@@ -132,7 +132,7 @@ __device__ __forceinline__ void ws_run(ws_u32x4 (&ring)[WS_RING], const CM_LDS u
 template <int G, int KT, int NT, bool GELU>
 __device__ __noinline__ void ws_layer(const CM_LDS uint16_t *A, int sa, const CM_GLB ws_u32x4 *W,
                                       const CM_GLB float *bias, CM_LDS void *out, int so, int wave, int lane) {
-  static_assert(G < NT, "every wave needs a first tile group");
+  static_assert(G <= NT, "every wave needs a first tile group");
   ws_u32x4 ring[WS_RING];
   const int n0 = wave + 2 * G < NT ? 3 : (wave + G < NT ? 2 : 1);
   if (n0 == 3)
@@ -169,13 +169,29 @@ __device__ __forceinline__ float rms256(float x0, float x1, float x2, float x3) 
   return __builtin_amdgcn_sqrtf(tree_sum(x0 * x0, x1 * x1, x2 * x2, x3 * x3) * (1.0f / 256.0f)) + 1e-30f;
 }
 
+// Section constants of a lockstep wave.  Every block of a launch has the same
+// section parameters, so the group keeps one table of them in LDS, one float4
+// per lane and row, filled once by wave 0; only the per-set gamma differs per
+// wave.  rsm and m (read by every sweep and residual check) and gamma are
+// copied into registers after each network evaluation; the other rows are
+// read from LDS where the pass uses them.  Nothing is re-read from global
+// memory and 1/mass is not recomputed per round.
+struct WsConst {
+  float rsm[4], mf[4], gam[4];
+  float low;
+  const CM_LDS float *R, *S, *Q, *Sohc, *Stm, *Rtm, *rmass;
+};
+constexpr int WS_TAB_ROWS = 9;   // R, S, Q, S_ohc, S_tm, R_tm, 1/mass, rsm, m
+
 template <int G>
 __global__ __launch_bounds__(64 * G) void bm_ws_kernel(BmLaunch a, WsArgs w) {
   __shared__ __align__(16) uint16_t Xs[16 * WS_XS];
   __shared__ __align__(16) uint16_t H1s[16 * WS_HS];
   __shared__ __align__(16) uint16_t H2s[16 * WS_HS];      // also the output tile (16 x WS_OS floats)
   __shared__ float4 pold_lds[G * 65];
-  __shared__ float4 keep_lds[G * 5 * 64];                  // predictor right-hand side across the network
+  __shared__ float4 keep_lds[G * 3 * 64];                  // predictor G, nxis, Ptm across the network
+  __shared__ float4 tab_lds[WS_TAB_ROWS * 64];
+  __shared__ float4 gam_lds[G * 64];                       // gamma of each wave's parameter set
   __shared__ int active_lds[2][G];                         // double-buffered by round parity
 
   const int lane = threadIdx.x & 63;
@@ -184,14 +200,14 @@ __global__ __launch_bounds__(64 * G) void bm_ws_kernel(BmLaunch a, WsArgs w) {
   const bool valid = slot < a.n_items;
   const int idx = valid ? (a.order ? a.order[slot] : slot) : slot;   // item integrated by this wave
   float4 *pold = pold_lds + wave * 65;
-  float4 *keep = keep_lds + wave * 5 * 64;   // Y, od, G, nxis, Ptm of the predictor pass
+  float4 *keep = keep_lds + wave * 3 * 64;   // G, nxis, Ptm of the predictor pass
   Lay<4> L{lane, 0, nullptr, 0};             // one wave per block
 
   // zero the feature tile once (rows >= G stay zero)
   for (int e = threadIdx.x; e < 16 * WS_XS; e += 64 * G) Xs[e] = 0;
 
   cm_bm_item it;
-  ItemConst k;
+  WsConst k;
   ItemScal c;
   ItemState s;
   if (valid) {
@@ -200,7 +216,29 @@ __global__ __launch_bounds__(64 * G) void bm_ws_kernel(BmLaunch a, WsArgs w) {
     it.start = 0; it.mosi = 0; it.mesi = 0; it.write_start = 0; it.tol = 0.f; it.m1tol = 0.f; it.set = 0;
     it.pad = 0;
   }
-  item_setup<true>(a, it, L, k, c, s);
+  {
+    ItemConst k0;
+    item_setup<true>(a, it, L, k0, c, s);
+    for (int j = 0; j < 4; ++j) { k.rsm[j] = k0.rsm[j]; k.mf[j] = k0.mf[j]; k.gam[j] = k0.gam[j]; }
+    k.low = k0.low;
+    if (wave == 0) {
+      tab_lds[0 * 64 + lane] = make_float4(k0.R[0], k0.R[1], k0.R[2], k0.R[3]);
+      tab_lds[1 * 64 + lane] = make_float4(k0.S[0], k0.S[1], k0.S[2], k0.S[3]);
+      tab_lds[2 * 64 + lane] = make_float4(k0.Q[0], k0.Q[1], k0.Q[2], k0.Q[3]);
+      tab_lds[3 * 64 + lane] = make_float4(k0.Sohc[0], k0.Sohc[1], k0.Sohc[2], k0.Sohc[3]);
+      tab_lds[4 * 64 + lane] = make_float4(k0.Stm[0], k0.Stm[1], k0.Stm[2], k0.Stm[3]);
+      tab_lds[5 * 64 + lane] = make_float4(k0.Rtm[0], k0.Rtm[1], k0.Rtm[2], k0.Rtm[3]);
+      tab_lds[6 * 64 + lane] = make_float4(k0.rmass[0], k0.rmass[1], k0.rmass[2], k0.rmass[3]);
+      tab_lds[7 * 64 + lane] = make_float4(k0.rsm[0], k0.rsm[1], k0.rsm[2], k0.rsm[3]);
+      tab_lds[8 * 64 + lane] = make_float4(k0.mf[0], k0.mf[1], k0.mf[2], k0.mf[3]);
+    }
+    gam_lds[wave * 64 + lane] = make_float4(k0.gam[0], k0.gam[1], k0.gam[2], k0.gam[3]);
+  }
+  {
+    const CM_LDS float *t = (const CM_LDS float *)(tab_lds) + 4 * lane;
+    k.R = t; k.S = t + 256; k.Q = t + 512; k.Sohc = t + 768; k.Stm = t + 1024; k.Rtm = t + 1280;
+    k.rmass = t + 1536;
+  }
   pold[lane] = make_float4(0.f, 0.f, 0.f, 0.f);
   if (lane == 0) reinterpret_cast<float *>(pold + 64)[0] = s.dt;
   __syncthreads();
@@ -215,11 +253,9 @@ __global__ __launch_bounds__(64 * G) void bm_ws_kernel(BmLaunch a, WsArgs w) {
       float od[4], G4[4], Y[4];
       pass_begin(s, true);
       const float sf = pass_rhs(k, c, s, true, L, od, G4, Y);
-      keep[lane] = make_float4(Y[0], Y[1], Y[2], Y[3]);
-      keep[64 + lane] = make_float4(od[0], od[1], od[2], od[3]);
-      keep[128 + lane] = make_float4(G4[0], G4[1], G4[2], G4[3]);
-      keep[192 + lane] = make_float4(s.nxis[0], s.nxis[1], s.nxis[2], s.nxis[3]);
-      keep[256 + lane] = make_float4(s.Ptm[0], s.Ptm[1], s.Ptm[2], s.Ptm[3]);
+      keep[lane] = make_float4(G4[0], G4[1], G4[2], G4[3]);
+      keep[64 + lane] = make_float4(s.nxis[0], s.nxis[1], s.nxis[2], s.nxis[3]);
+      keep[128 + lane] = make_float4(s.Ptm[0], s.Ptm[1], s.Ptm[2], s.Ptm[3]);
       if (lane == 0) reinterpret_cast<float *>(pold + 64)[1] = sf;
       rr0 = extrapolate_guess(L, k, Y, s.dt, pold, s.p, rre, true);   // s.p = p_lin
       rr_scale = __builtin_amdgcn_sqrtf(rre * inv_256) + 1e-30f;
@@ -285,27 +321,37 @@ __global__ __launch_bounds__(64 * G) void bm_ws_kernel(BmLaunch a, WsArgs w) {
       ws_layer<G, 32, 16, false>((lds_in)H1s, WS_HS, (glb_w)w.w3, (glb_b)w.b3, (lds_out)H2s, WS_OS, wave, lane);
       __syncthreads();
     }
-    // The section constants are re-read from L1/L2 here, so that they are not
-    // held in registers across the network evaluation (the memory clobber keeps
-    // the compiler from reusing the earlier loads).
+    // rsm, m and gamma are copied back from LDS here, so that they are not held
+    // in registers across the network evaluation (the memory clobber keeps the
+    // compiler from reusing the earlier loads).
     asm volatile("" ::: "memory");
-    load_consts<true>(a, it, L, k);
+    {
+      const float4 r4 = tab_lds[7 * 64 + lane], m4 = tab_lds[8 * 64 + lane], g4 = gam_lds[wave * 64 + lane];
+      k.rsm[0] = r4.x; k.rsm[1] = r4.y; k.rsm[2] = r4.z; k.rsm[3] = r4.w;
+      k.mf[0] = m4.x; k.mf[1] = m4.y; k.mf[2] = m4.z; k.mf[3] = m4.w;
+      k.gam[0] = g4.x; k.gam[1] = g4.y; k.gam[2] = g4.z; k.gam[3] = g4.w;
+    }
     if (active) {
-      // the predictor state again (same inputs, same values) and its
-      // right-hand side from LDS: cheaper than holding either in registers
-      // across the network
+      // the predictor state again (same inputs, same values), G, nxis and Ptm
+      // from LDS, and od and Y formed from them as pass_rhs formed them (same
+      // operations on the same values): cheaper than holding any of it in
+      // registers across the network
       float od[4], G4[4], Y[4];
       pass_begin(s, true);
+      const float sf = reinterpret_cast<const float *>(pold + 64)[1];
       {
-        const float4 y = keep[lane], o = keep[64 + lane], g = keep[128 + lane], x = keep[192 + lane],
-                     t = keep[256 + lane];
-        Y[0] = y.x; Y[1] = y.y; Y[2] = y.z; Y[3] = y.w;
-        od[0] = o.x; od[1] = o.y; od[2] = o.z; od[3] = o.w;
+        const float4 g = keep[lane], x = keep[64 + lane], t = keep[128 + lane];
         G4[0] = g.x; G4[1] = g.y; G4[2] = g.z; G4[3] = g.w;
         s.nxis[0] = x.x; s.nxis[1] = x.y; s.nxis[2] = x.z; s.nxis[3] = x.w;
         s.Ptm[0] = t.x; s.Ptm[1] = t.y; s.Ptm[2] = t.z; s.Ptm[3] = t.w;
+        for (int j = 0; j < 4; ++j) {
+          od[j] = s.nxi[j] - s.nd[j];
+          Y[j] = (c.dxdx * G4[j]) * k.Q[j];
+        }
+        const float Y0 = c.dxa0 * (((c.neg_a2 * s.ow_nv) - c.a1 * s.ow_nd) + c.Gme * sf);
+        if (lane == 0) Y[0] = Y0;
+        if (lane == 63) Y[3] = 0.0f;
       }
-      const float sf = reinterpret_cast<const float *>(pold + 64)[1];
       const float yy = tree_sum(Y[0] * Y[0], Y[1] * Y[1], Y[2] * Y[2], Y[3] * Y[3]);
       float rr_sel = rre;   // residual of the guess kept (p_lin unless replaced below)
       if (net_round) {
@@ -414,21 +460,31 @@ extern "C" int cm_launch_bm(const cm::BmLaunch *a, int waves_per_block, int unro
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
-// Lockstep groups of G blocks, G = 8 when the groups fit one per CU, else 12.
-// A round costs the solver work of the busiest SIMD (the waves of a group are
-// spread over the CU's 4 SIMDs) plus the network's weight stream, whose time
-// grows with the number of CUs streaming from L2 at once.  G = 8 puts 2 waves
-// on every SIMD; G = 12 puts 3 on every SIMD, so G = 10 (3, 3, 2, 2) costs the
-// same solver time as 12 with more groups streaming weights, and is not used.
+// Lockstep groups of G in {8, 12, 16} blocks.  A round costs the solver work
+// of the busiest SIMD (a group's waves are spread over the CU's 4 SIMDs: 2, 3
+// or 4 per SIMD) plus the network's weight stream, which is the same for every
+// G; so bigger groups stream fewer bytes per block-step but take longer per
+// round.  Measured per wave of groups over the chip (the launch time over
+// ceil(n / (G x CUs))): G = 8 costs 0.9 and G = 16 1.25 times G = 12.  The
+// host picks the G with the smallest ceil(n / (G x CUs)) x cost (ties: the
+// smaller G), so small launches fill the CUs with small groups and large
+// ones use 16; cm_ws_set_group fixes G instead.  Results do not depend on G.
 // Returns G (> 0) or a negative error.
 extern "C" int cm_launch_bm_ws(const cm::BmLaunch *a, const cm::WsArgs *w, int n_cus, hipStream_t stream) {
   if (a->sc.jacobi_mode != 1 || w->net_every < 1) return -1;
   const int n = a->n_items;
-  const int G = (n + 7) / 8 <= n_cus ? 8 : 12;
+  const int cus = n_cus > 0 ? n_cus : 1;
+  const int64_t c8 = 18 * (int64_t)((n + 8 * cus - 1) / (8 * cus));
+  const int64_t c12 = 20 * (int64_t)((n + 12 * cus - 1) / (12 * cus));
+  const int64_t c16 = 25 * (int64_t)((n + 16 * cus - 1) / (16 * cus));
+  const int G = w->group ? w->group : ((c8 <= c12 && c8 <= c16) ? 8 : (c12 <= c16 ? 12 : 16));
+  if (G != 8 && G != 12 && G != 16) return -1;
   const dim3 grid((n + G - 1) / G);
   if (G == 8)
     hipLaunchKernelGGL((cm::bm_ws_kernel<8>), grid, dim3(64 * 8), 0, stream, *a, *w);
-  else
+  else if (G == 12)
     hipLaunchKernelGGL((cm::bm_ws_kernel<12>), grid, dim3(64 * 12), 0, stream, *a, *w);
+  else
+    hipLaunchKernelGGL((cm::bm_ws_kernel<16>), grid, dim3(64 * 16), 0, stream, *a, *w);
   return hipGetLastError() == hipSuccess ? G : -2;
 }

@@ -47,6 +47,7 @@ struct WsArgs {
   int32_t stride;                    // capture every stride-th committed step
   int32_t per_item;                  // capture slots per item
   int32_t net_every;                 // evaluate the network on every net_every-th round (>= 1)
+  int32_t group;                     // lockstep group size 8, 12 or 16; 0: chosen from the launch size
 };
 
 // record layout (floats): features [0, 769), rms(r_lin) at 769, committed step

@@ -23,6 +23,7 @@ struct WsHolder {
   int64_t n_records = 0;
   int stride = 0, per_item = 0;
   int net_every = 1;          // network on every net_every-th lockstep round
+  int group = 0;              // lockstep group size 8, 12 or 16; 0: from the launch size
   double *ainv = nullptr;     // 256 x 256 inverse of the kernel-convention matrix (targets)
 };
 WsHolder g_ws;
@@ -104,6 +105,7 @@ int ws_args(const cm_bm_scalars &sc, int n_items, WsArgs *out) {
     out->use_net = 1;
   }
   out->net_every = g_ws.net_every;
+  out->group = g_ws.group;
   if (g_ws.records) {
     if ((int64_t)n_items * g_ws.per_item > g_ws.n_records) {
       set_error("capture buffer holds %lld records, %d items x %d slots need more", (long long)g_ws.n_records,
@@ -184,6 +186,15 @@ int cm_ws_load_file(const char *path) {
 // Evaluate the warm-start network on every k-th lockstep round only; the other
 // rounds take the better of the previous and the extrapolated pressure, as
 // warm_start = 1 does.  k = 1 (default) uses the network on every step.
+int cm_ws_set_group(int g) {
+  if (g != 0 && g != 8 && g != 12 && g != 16) {
+    set_error("cm_ws_set_group: 0 (from the launch size), 8, 12 or 16 (got %d)", g);
+    return -3;
+  }
+  g_ws.group = g;
+  return 0;
+}
+
 int cm_ws_set_every(int k) {
   if (k < 1 || k > 1000000) {
     set_error("cm_ws_set_every: k must be >= 1");

@@ -380,16 +380,20 @@ int cm_mse_grad(const float *Y, const float *T, float *dY, void *dYbf, float *ro
  * bf16 MFMA fragments on the host), biases fp32, target scale c_out. */
 int cm_ws_load(const float *W1, const float *b1, const float *W2, const float *b2, const float *W3, const float *b3,
                float c_out);
+/* run the warm-start network on every k-th lockstep round only (k >= 1; 1 = every
+ * step); other rounds take the better of the previous and extrapolated pressure
+ * (no reference counterpart) */
+int cm_ws_set_every(int k);
+/* Lockstep group size of the learned kernel (no reference counterpart): 8, 12
+ * or 16 blocks per workgroup, or 0 (default) to choose it from the launch
+ * size.  Results do not depend on it. */
+int cm_ws_set_group(int g);
 /* Training records: with warm_start 1 or 2 (jacobi_mode 1) every cm_bm_run
  * also writes, for every stride-th committed step of item i (slot
  * i * per_item + step / stride), the network input (769 floats at 0),
  * rms(r_lin) (769), the step index (770), Y (256 at 772), p_lin (256 at 1028)
  * and, with warm_start 2, the network output (256 at 1288); 1544 floats per
  * record.  d_records = NULL switches capture off. */
-/* run the warm-start network on every k-th lockstep round only (k >= 1; 1 = every
- * step); other rounds take the better of the previous and extrapolated pressure
- * (no reference counterpart) */
-int cm_ws_set_every(int k);
 int cm_ws_set_capture(float *d_records, int64_t n_records, int stride, int per_item);
 /* cm_ws_load from a file of float32: W1, b1, W2, b2, W3, b3 (as above) and c_out */
 int cm_ws_load_file(const char *path);

@@ -87,6 +87,9 @@ def test_bm_run_rejects_invalid_arguments_before_any_launch():
     assert L.cm_set_bm_jacobi_unroll(3) == -3
     for n in (1, 2, 4):
         assert L.cm_set_bm_sections_per_lane(n) == 0
+    assert L.cm_ws_set_group(10) == -3 and "8, 12 or 16" in msg()
+    for g in (8, 12, 16, 0):
+        assert L.cm_ws_set_group(g) == 0
 
 
 def test_batched_stage_entry_points_reject_bad_sizes_before_any_launch():

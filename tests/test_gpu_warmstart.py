@@ -122,13 +122,15 @@ def test_exact_targets_match_dense_solve():
 
 
 @pytest.mark.parametrize("n,dur,per_item,group", [(4, 0.04, 64, 8), (2040, 0.02, 4, 8), (2200, 0.02, 4, 12),
-                                                   (2700, 0.02, 4, 12), (3300, 0.02, 4, 12)])
+                                                   (2700, 0.02, 4, 12), (3300, 0.02, 4, 16),
+                                                   (6200, 0.02, 4, 16)])
 def test_in_solver_network_matches_oracle(n, dur, per_item, group):
     """The MFMA inference inside the solver (fragment packing, tile mapping,
     GELU, bf16 operands) against the numpy oracle on the captured inputs, for
-    both lockstep group sizes (8 blocks when the groups fit one per CU of 256,
-    else 12), with full and partly filled last groups, and more groups than
-    CUs (3300 blocks: 275 groups)."""
+    the three lockstep group sizes the launch picks on 256 CUs (8 while groups
+    of 8 fit one per CU, 12 for one wave of groups of 12, 16 beyond), with full
+    and partly filled last groups, and more groups than CUs (6200 blocks: 388
+    groups of 16)."""
     _cuda()
     from cochlearmodel_amd import warmstart
     from cochlearmodel_amd.mlp import SIZES
@@ -260,3 +262,30 @@ def test_learned_solver_matches_its_cpu_port(oracle):
     assert abs(g_steps - c_steps) <= 0.01 * c_steps
     assert abs(g_sweeps / g_steps - c_sweeps / c_steps) <= 0.02 * (c_sweeps / c_steps)
     assert d <= 3e-4
+
+
+def test_group_size_does_not_change_results():
+    """Groups of 8, 12 and 16 blocks (cm_ws_set_group) give bit-identical
+    speeds and counters: every block integrates on its own, the network's rows
+    are independent, and the group only decides which blocks share a weight
+    stream.  50 blocks leave a partly filled last group at every size."""
+    _cuda()
+    from cochlearmodel_amd import _lib, warmstart
+    params, c_out, _ = warmstart.load_model()
+    warmstart.upload(params, c_out)
+    _lib.check(_lib.lib().cm_ws_set_every(1))
+    _say("case: learned warm start, 50 blocks, groups of 8, 12 and 16")
+    tb = _batch(n=10, dur=0.1, seed=29)
+    assert tb.TB == 50
+    sc = warmstart.tol_scalars(1e-6, 2)
+    L = _lib.lib()
+    out = {}
+    try:
+        for g in (8, 12, 16):
+            _lib.check(L.cm_ws_set_group(g))
+            out[g] = _run(tb, sc)
+    finally:
+        _lib.check(L.cm_ws_set_group(0))
+    for g in (12, 16):
+        np.testing.assert_array_equal(out[g][0], out[8][0], err_msg=f"group {g}")
+        np.testing.assert_array_equal(out[g][1], out[8][1], err_msg=f"group {g}")
