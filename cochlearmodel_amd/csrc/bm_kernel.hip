@@ -133,32 +133,38 @@ template <int G, int KT, int NT, bool GELU>
 __device__ __noinline__ void ws_layer(const CM_LDS uint16_t *A, int sa, const CM_GLB ws_u32x4 *W,
                                       const CM_GLB float *bias, CM_LDS void *out, int so, int wave, int lane) {
   static_assert(G <= NT, "every wave needs a first tile group");
+  // tile counts a wave can have: 3 only when NT > 2G, 2 only when NT > G (the
+  // others are not compiled, so the ring and accumulators are sized for what
+  // this G can use)
+  constexpr bool may3 = NT > 2 * G, may2 = NT > G;
   ws_u32x4 ring[WS_RING];
-  const int n0 = wave + 2 * G < NT ? 3 : (wave + G < NT ? 2 : 1);
-  if (n0 == 3)
-    ws_fill<G, KT, NT, 3>(ring, W, lane, wave);
-  else if (n0 == 2)
-    ws_fill<G, KT, NT, 2>(ring, W, lane, wave);
+  const int n0 = may3 && wave + 2 * G < NT ? 3 : (may2 && wave + G < NT ? 2 : 1);
+  if (may3 && n0 == 3)
+    ws_fill<G, KT, NT, (may3 ? 3 : 1)>(ring, W, lane, wave);
+  else if (may2 && n0 == 2)
+    ws_fill<G, KT, NT, (may2 ? 2 : 1)>(ring, W, lane, wave);
   else
     ws_fill<G, KT, NT, 1>(ring, W, lane, wave);
   __syncthreads();
-  if (n0 == 3)
-    ws_run<G, KT, NT, GELU, 3>(ring, A, sa, W, bias, out, so, lane, wave);
-  else if (n0 == 2)
-    ws_run<G, KT, NT, GELU, 2>(ring, A, sa, W, bias, out, so, lane, wave);
+  if (may3 && n0 == 3)
+    ws_run<G, KT, NT, GELU, (may3 ? 3 : 1)>(ring, A, sa, W, bias, out, so, lane, wave);
+  else if (may2 && n0 == 2)
+    ws_run<G, KT, NT, GELU, (may2 ? 2 : 1)>(ring, A, sa, W, bias, out, so, lane, wave);
   else
     ws_run<G, KT, NT, GELU, 1>(ring, A, sa, W, bias, out, so, lane, wave);
-  for (int t0 = wave + 3 * G; t0 < NT; t0 += 3 * G) {
-    const int n = t0 + 2 * G < NT ? 3 : (t0 + G < NT ? 2 : 1);
-    if (n == 3) {
-      ws_fill<G, KT, NT, 3>(ring, W, lane, t0);
-      ws_run<G, KT, NT, GELU, 3>(ring, A, sa, W, bias, out, so, lane, t0);
-    } else if (n == 2) {
-      ws_fill<G, KT, NT, 2>(ring, W, lane, t0);
-      ws_run<G, KT, NT, GELU, 2>(ring, A, sa, W, bias, out, so, lane, t0);
-    } else {
-      ws_fill<G, KT, NT, 1>(ring, W, lane, t0);
-      ws_run<G, KT, NT, GELU, 1>(ring, A, sa, W, bias, out, so, lane, t0);
+  if constexpr (NT > 3 * G) {
+    for (int t0 = wave + 3 * G; t0 < NT; t0 += 3 * G) {
+      const int n = t0 + 2 * G < NT ? 3 : (t0 + G < NT ? 2 : 1);
+      if (n == 3) {
+        ws_fill<G, KT, NT, 3>(ring, W, lane, t0);
+        ws_run<G, KT, NT, GELU, 3>(ring, A, sa, W, bias, out, so, lane, t0);
+      } else if (n == 2) {
+        ws_fill<G, KT, NT, 2>(ring, W, lane, t0);
+        ws_run<G, KT, NT, GELU, 2>(ring, A, sa, W, bias, out, so, lane, t0);
+      } else {
+        ws_fill<G, KT, NT, 1>(ring, W, lane, t0);
+        ws_run<G, KT, NT, GELU, 1>(ring, A, sa, W, bias, out, so, lane, t0);
+      }
     }
   }
 }
