@@ -471,18 +471,19 @@ extern "C" int cm_launch_bm(const cm::BmLaunch *a, int waves_per_block, int unro
 // or 4 per SIMD) plus the network's weight stream, which is the same for every
 // G; so bigger groups stream fewer bytes per block-step but take longer per
 // round.  Measured per wave of groups over the chip (the launch time over
-// ceil(n / (G x CUs))): G = 8 costs 0.9 and G = 16 1.25 times G = 12.  The
-// host picks the G with the smallest ceil(n / (G x CUs)) x cost (ties: the
-// smaller G), so small launches fill the CUs with small groups and large
-// ones use 16; cm_ws_set_group fixes G instead.  Results do not depend on G.
+// ceil(n / (G x CUs)) on 20480 held-out blocks): G = 8 costs 0.95 and G = 16
+// 1.2 times G = 12.  The host picks the G with the smallest
+// ceil(n / (G x CUs)) x cost (ties: the smaller G), so small launches fill the
+// CUs with small groups and large ones use 16; cm_ws_set_group fixes G
+// instead.  Results do not depend on G.
 // Returns G (> 0) or a negative error.
 extern "C" int cm_launch_bm_ws(const cm::BmLaunch *a, const cm::WsArgs *w, int n_cus, hipStream_t stream) {
   if (a->sc.jacobi_mode != 1 || w->net_every < 1) return -1;
   const int n = a->n_items;
   const int cus = n_cus > 0 ? n_cus : 1;
-  const int64_t c8 = 18 * (int64_t)((n + 8 * cus - 1) / (8 * cus));
+  const int64_t c8 = 19 * (int64_t)((n + 8 * cus - 1) / (8 * cus));
   const int64_t c12 = 20 * (int64_t)((n + 12 * cus - 1) / (12 * cus));
-  const int64_t c16 = 25 * (int64_t)((n + 16 * cus - 1) / (16 * cus));
+  const int64_t c16 = 24 * (int64_t)((n + 16 * cus - 1) / (16 * cus));
   const int G = w->group ? w->group : ((c8 <= c12 && c8 <= c16) ? 8 : (c12 <= c16 ? 12 : 16));
   if (G != 8 && G != 12 && G != 16) return -1;
   const dim3 grid((n + G - 1) / G);
