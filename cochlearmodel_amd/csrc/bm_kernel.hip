@@ -183,9 +183,8 @@ __device__ __forceinline__ float rms256(float x0, float x1, float x2, float x3) 
 // read from LDS where the pass uses them.  Nothing is re-read from global
 // memory and 1/mass is not recomputed per round.
 struct WsConst {
-  float rsm[4], mf[4], gam[4];
   float low;
-  const CM_LDS float *R, *S, *Q, *Sohc, *Stm, *Rtm, *rmass;
+  const CM_LDS float *R, *S, *Q, *Sohc, *Stm, *Rtm, *rmass, *rsm, *mf, *gam;
 };
 constexpr int WS_TAB_ROWS = 9;   // R, S, Q, S_ohc, S_tm, R_tm, 1/mass, rsm, m
 
@@ -204,7 +203,7 @@ __global__ __launch_bounds__(64 * G) void bm_ws_kernel(BmLaunch a, WsArgs w) {
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int slot = blockIdx.x * G + wave;
   const bool valid = slot < a.n_items;
-  const int idx = valid ? (a.order ? a.order[slot] : slot) : slot;   // item integrated by this wave
+  const int idx = __builtin_amdgcn_readfirstlane(valid ? (a.order ? a.order[slot] : slot) : slot);   // item of this wave
   float4 *pold = pold_lds + wave * 65;
   float4 *keep = keep_lds + wave * 3 * 64;   // G, nxis, Ptm of the predictor pass
   Lay<4> L{lane, 0, nullptr, 0};             // one wave per block
@@ -217,7 +216,18 @@ __global__ __launch_bounds__(64 * G) void bm_ws_kernel(BmLaunch a, WsArgs w) {
   ItemScal c;
   ItemState s;
   if (valid) {
-    it = a.items[idx];
+    // the record is the same for every lane: its fields go to scalar registers
+    // (a vector load would leave them, and the addresses formed from them, in
+    // VGPRs that the groups of 16 cannot spare)
+    const cm_bm_item r = a.items[idx];
+    it.start = __builtin_amdgcn_readfirstlane(r.start);
+    it.mosi = __builtin_amdgcn_readfirstlane(r.mosi);
+    it.mesi = __builtin_amdgcn_readfirstlane(r.mesi);
+    it.write_start = __builtin_amdgcn_readfirstlane(r.write_start);
+    it.tol = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(r.tol)));
+    it.m1tol = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(r.m1tol)));
+    it.set = __builtin_amdgcn_readfirstlane(r.set);
+    it.pad = 0;
   } else {
     it.start = 0; it.mosi = 0; it.mesi = 0; it.write_start = 0; it.tol = 0.f; it.m1tol = 0.f; it.set = 0;
     it.pad = 0;
@@ -225,7 +235,6 @@ __global__ __launch_bounds__(64 * G) void bm_ws_kernel(BmLaunch a, WsArgs w) {
   {
     ItemConst k0;
     item_setup<true>(a, it, L, k0, c, s);
-    for (int j = 0; j < 4; ++j) { k.rsm[j] = k0.rsm[j]; k.mf[j] = k0.mf[j]; k.gam[j] = k0.gam[j]; }
     k.low = k0.low;
     if (wave == 0) {
       tab_lds[0 * 64 + lane] = make_float4(k0.R[0], k0.R[1], k0.R[2], k0.R[3]);
@@ -243,7 +252,8 @@ __global__ __launch_bounds__(64 * G) void bm_ws_kernel(BmLaunch a, WsArgs w) {
   {
     const CM_LDS float *t = (const CM_LDS float *)(tab_lds) + 4 * lane;
     k.R = t; k.S = t + 256; k.Q = t + 512; k.Sohc = t + 768; k.Stm = t + 1024; k.Rtm = t + 1280;
-    k.rmass = t + 1536;
+    k.rmass = t + 1536; k.rsm = t + 1792; k.mf = t + 2048;
+    k.gam = (const CM_LDS float *)(gam_lds + wave * 64 + lane);
   }
   pold[lane] = make_float4(0.f, 0.f, 0.f, 0.f);
   if (lane == 0) reinterpret_cast<float *>(pold + 64)[0] = s.dt;
@@ -326,16 +336,6 @@ __global__ __launch_bounds__(64 * G) void bm_ws_kernel(BmLaunch a, WsArgs w) {
       ws_layer<G, 32, 32, true>((lds_in)H2s, WS_HS, (glb_w)w.w2, (glb_b)w.b2, (lds_out)H1s, WS_HS, wave, lane);
       ws_layer<G, 32, 16, false>((lds_in)H1s, WS_HS, (glb_w)w.w3, (glb_b)w.b3, (lds_out)H2s, WS_OS, wave, lane);
       __syncthreads();
-    }
-    // rsm, m and gamma are copied back from LDS here, so that they are not held
-    // in registers across the network evaluation (the memory clobber keeps the
-    // compiler from reusing the earlier loads).
-    asm volatile("" ::: "memory");
-    {
-      const float4 r4 = tab_lds[7 * 64 + lane], m4 = tab_lds[8 * 64 + lane], g4 = gam_lds[wave * 64 + lane];
-      k.rsm[0] = r4.x; k.rsm[1] = r4.y; k.rsm[2] = r4.z; k.rsm[3] = r4.w;
-      k.mf[0] = m4.x; k.mf[1] = m4.y; k.mf[2] = m4.z; k.mf[3] = m4.w;
-      k.gam[0] = g4.x; k.gam[1] = g4.y; k.gam[2] = g4.z; k.gam[3] = g4.w;
     }
     if (active) {
       // the predictor state again (same inputs, same values), G, nxis and Ptm
