@@ -79,7 +79,7 @@ MLP_FLOP_PER_STEP = 2.0 * (769 * 512 + 512 * 512 + 512 * 256)   # forward, one b
 # WRITE_SIZE (KiB per dispatch), scaled by what the counters read for a 1 GiB
 # device copy (FETCH x 2.0, WRITE x 1.0); profiles/pmc_traffic_c2.md
 MEASURED_HBM_BYTES = {
-    "learned_tol": (12244.125 * 2.0 + 2258200.0) * 1024,
+    "learned_tol": (12239.25 * 2.0 + 2258200.0) * 1024,
     "reference_20_2": (6085.375 * 2.0 + 2258260.0) * 1024,
     "ihc_lambda": (1150991.3125 * 2.0 + 4515840.0 + 2288920.5 * 2.0 + 6773760.0) * 1024,
 }
@@ -88,7 +88,7 @@ PEAK_BF16_TFLOPS = 2500.0    # MI355X dense BF16 MFMA
 # profiles/pmc_valu_c2_reference.md, profiles/pmc_mix_c2_learned.md) and the issue
 # capacity they are measured against: a wave64 FP32 VALU instruction holds a SIMD
 # for 4 cycles, 1024 SIMDs at 2.4 GHz
-MEASURED_VALU_INSTS = {"reference_20_2": 1.0995e11, "learned_tol": 1.64287257572e11,
+MEASURED_VALU_INSTS = {"reference_20_2": 1.0995e11, "learned_tol": 1.64154213976e11,
                        "ihc_lambda": 1.40262e9 + 2.68728e9}   # ac_ds_kernel + dc_lambda_kernel
 # VALU issue capacity: the rate bm_kernel's own instruction mix reaches at full
 # occupancy (SQ_INSTS_VALU / kernel time on the JND sweep's 48300 blocks,
@@ -107,16 +107,16 @@ PEAK_FP32_UNPACKED_TFLOPS = PEAK_FP32_TFLOPS / 2
 # longest block first; profiles/pmc_traffic_c5_learned.md, profiles/pmc_mix_c5_learned.md)
 C5_MEASURED = {
     "tones_per_launch": 2048,
-    "hbm_bytes": {"learned_tol": (907491531.6875 * 2.0 + 1233402402.59375) * 1024,
+    "hbm_bytes": {"learned_tol": (2650581.8125 * 2.0 + 668347909.9375) * 1024,
                   "ihc_lambda": (9206258.75 * 2.0 + 36126720.0 + 18361109.0625 * 2.0 + 54190080.0) * 1024},
-    "valu_insts": {"learned_tol": 1.234939646876e12, "plain_tol": 1.269936053668e12,
+    "valu_insts": {"learned_tol": 1.230589350574e12, "plain_tol": 1.269936053668e12,
                    "ihc_lambda": 1.1221170816e10 + 2.1498514912e10},
-    "l2_read_bytes": {"learned_tol": 4.09001455453e11 * 128},
+    "l2_read_bytes": {"learned_tol": 3.90401022744e11 * 128},
     "mfma_insts": {"learned_tol": 4.769789056e10},
 }
 # L1<-L2 reads of the learned kernel: TCP_TCC_READ_REQ_sum x 128 B (a 1 GiB copy
 # reads 8.389e6 requests); profiles/pmc_mix_c2_learned.md
-MEASURED_L2_READ_BYTES = {"learned_tol": 6.5295413631e10 * 128}
+MEASURED_L2_READ_BYTES = {"learned_tol": 6.5295415974e10 * 128}
 
 
 def load_warm_start_model():
@@ -696,9 +696,9 @@ def c5_main(args):
                                            "on a 1 GiB copy); algorithmic: input 72 MB + velocity 18.5 GB")
         if args.variant == "learned_tol":
             out["roofline"]["traffic_note"] += (
-                "; the learned kernel's groups of 16 (4 waves per SIMD, 128 VGPRs) spill 128 B per lane, whose "
-                "stores and reloads overflow the 4 MB L2 of each XCD: almost all of the traffic, and still the "
-                "fastest form of the launch (spill-free groups of 12 move 20 GB but take 16% longer)")
+                "; the learned kernel's groups of 16 (4 waves per SIMD, 128 VGPRs) spill 64 B per lane: the "
+                "reloads hit the 4 MB L2 of each XCD (2 MiB of spill slots per XCD), and the spill stores make "
+                "almost all of the write traffic")
     if args.variant in M["valu_insts"]:
         out["roofline"]["valu_issue"] = {
             "insts_per_launch": M["valu_insts"][args.variant],
