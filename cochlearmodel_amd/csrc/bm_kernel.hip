@@ -178,10 +178,10 @@ __device__ __forceinline__ float rms256(float x0, float x1, float x2, float x3) 
 // Section constants of a lockstep wave.  Every block of a launch has the same
 // section parameters, so the group keeps one table of them in LDS, one float4
 // per lane and row, filled once by wave 0; only the per-set gamma differs per
-// wave.  rsm and m (read by every sweep and residual check) and gamma are
-// copied into registers after each network evaluation; the other rows are
-// read from LDS where the pass uses them.  Nothing is re-read from global
-// memory and 1/mass is not recomputed per round.
+// wave.  Every row, gamma included, is read from LDS where a pass uses it
+// (the compiler keeps a row in registers only through the loops that read
+// it, so nothing constant is held across the network evaluation).  Nothing
+// is re-read from global memory and 1/mass is not recomputed per round.
 struct WsConst {
   float low;
   const CM_LDS float *R, *S, *Q, *Sohc, *Stm, *Rtm, *rmass, *rsm, *mf, *gam;
