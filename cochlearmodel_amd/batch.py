@@ -57,6 +57,7 @@ class ToneBatch:
         self.d_speeds = torch.zeros((self.bufP1, SECTIONS), dtype=torch.float32, device=dev)
         self.d_stats = torch.zeros(self.TB * 3, dtype=torch.int64, device=dev)
         self.with_lambda = with_lambda
+        self.order = None   # launch order installed before each BM launch of this batch (cm_set_bm_order)
         if with_lambda:
             self.ihc = stages.BatchedIHC(FS)
             self.fir_ivs, _ = stages.reference_intervals(self.rows, self.block, self.TB, self.DF)
@@ -86,6 +87,8 @@ class ToneBatch:
     def run_bm(self, sc, timed: bool = True):
         L, lib = self.L, _lib
         L.cm_set_stream(self.stream)
+        if self.order is not None:
+            lib.check(L.cm_set_bm_order(lib.ptr(self.order), len(self.order)), "order")
         lib.check(L.cm_bm_run(C.byref(sc), lib.ptr(self.d_input), self.d_input.numel(), lib.ptr(self.bm.sec),
                               lib.ptr(self.bm.gamma), lib.ptr(self.d_items), self.TB, lib.ptr(self.d_speeds),
                               self.bufP1, None, None, None, None, None, None, None, None, lib.ptr(self.d_stats), 1, 0,
@@ -115,12 +118,24 @@ class ToneBatch:
 
     def longest_first_order(self) -> np.ndarray:
         """Launch order of the block table, longest block first (ties in table
-        order): the first block of every tone is the shortest, so in natural
-        order each lockstep group mixes lengths and the last wave of work
-        holds full-length blocks.  For cm_set_bm_order; results do not depend
+        order): the last block of every tone is the shortest (it ends at the
+        tone's boundary without the overlap run-out), so in natural order each
+        lockstep group mixes lengths and the last wave of work holds
+        full-length blocks.  For cm_set_bm_order; results do not depend
         on it."""
         lens = self.items["mesi"].astype(np.int64) - self.items["start"]
         return np.argsort(-lens, kind="stable").astype(np.int32)
+
+    def tone_order(self) -> np.ndarray:
+        """Longest block first, then the blocks of higher-frequency tones first
+        (ties in table order).  With a guess (extrapolated or learned) a tone's
+        Jacobi sweeps per step follow its frequency closely (more sweeps at
+        higher frequency), so lockstep groups of neighbouring frequencies wait
+        less for their slowest block at each round, and the heaviest groups are
+        dealt out first.  For cm_set_bm_order; results do not depend on it."""
+        lens = self.items["mesi"].astype(np.int64) - self.items["start"]
+        tone = np.arange(self.TB) // self.DF
+        return np.lexsort((np.arange(self.TB), -self.freqs[tone], -lens)).astype(np.int32)
 
     def stats(self):
         s = self.d_stats.view(self.TB, 3).cpu().numpy().astype(np.int64)
