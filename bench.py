@@ -74,11 +74,15 @@ def set_network_every(name: str):
 
 VARIANTS = ("reference_20_2", "plain_tol", "extrapolated_tol", "learned_tol")
 EXTRA_VARIANTS = ("learned_tol_every2", "learned_tol_every4")   # reported in the variants table only
-# variants launched in ToneBatch.tone_order on the held-out set (longest block
-# first, then higher-frequency tones first); the others keep longest-first.
-# Measured on a 2048-tone launch (ms, longest-first -> tone order, identical
-# results): learned 3071 -> 2937, extrapolated 1312 -> 1253, plain 1781 -> 1800
-TONE_ORDER_VARIANTS = ("learned_tol", "learned_tol_every2", "learned_tol_every4", "extrapolated_tol")
+# Launch order of each variant on the held-out set, heavy tones first
+# (ToneBatch.tone_order): with a guess, sweeps per step rise with a tone's
+# frequency, without one they fall.  Measured on a 2048-tone launch (ms,
+# longest-first -> this order, identical results): learned 3071 -> 2937 and
+# extrapolated 1312 -> 1253 (higher frequency first), plain 1783 -> 1745 (lower
+# frequency first).  The reference method (fixed sweeps) keeps longest-first:
+# 1145-1156 ms in every order.
+TONE_ORDER = {"learned_tol": "high", "learned_tol_every2": "high", "learned_tol_every4": "high",
+              "extrapolated_tol": "high", "plain_tol": "low"}
 MLP_FLOP_PER_STEP = 2.0 * (769 * 512 + 512 * 512 + 512 * 256)   # forward, one block-step
 # HBM bytes per launch on the 256-tone batch from rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE (KiB per dispatch), scaled by what the counters read for a 1 GiB
@@ -625,14 +629,15 @@ def c5_main(args):
         tbs.append(tb)
     rows0 = tbs[0].rows
     nb = len(tbs)
-    # launch order, per batch (each launch installs its own): longest block
-    # first; the methods with a guess then take higher-frequency tones first
+    # launch order per variant and batch (each launch installs its own):
+    # longest block first, then the variant's heavy tones first
     from cochlearmodel_amd import _lib
-    orders = [(tb.longest_first_order(), tb.tone_order()) for tb in tbs]
+    orders = [{None: tb.longest_first_order(), "high": tb.tone_order(True), "low": tb.tone_order(False)}
+              for tb in tbs]
 
     def use_orders(name):
-        for tb, (lf, to) in zip(tbs, orders):
-            tb.order = to if name in TONE_ORDER_VARIANTS else lf
+        for tb, o in zip(tbs, orders):
+            tb.order = o[TONE_ORDER.get(name)]
 
     use_orders(args.variant)
     model_meta = load_warm_start_model()
@@ -678,8 +683,10 @@ def c5_main(args):
                                "Decouple_Filter=10, all four outputs resident; one step = input synthesis + BM + "
                                "IHC/lambda of one launch",
                    "variant": args.variant,
-                   "launch_order": ("longest block first, then higher-frequency tones first"
-                                    if args.variant in TONE_ORDER_VARIANTS else "longest block first"),
+                   "launch_order": {None: "longest block first",
+                                    "high": "longest block first, then higher-frequency tones first",
+                                    "low": "longest block first, then lower-frequency tones first"}[
+                                        TONE_ORDER.get(args.variant)],
                    "signals": nb * tbs[0].n, "signals_per_step": tbs[0].n,
                    "time_blocks_per_step": tbs[0].TB, "rows_per_step": rows0},
         "jacobi_iters_per_step": sweeps / steps, "passes_per_step": passes / steps, "committed_steps": steps,

@@ -126,16 +126,19 @@ class ToneBatch:
         lens = self.items["mesi"].astype(np.int64) - self.items["start"]
         return np.argsort(-lens, kind="stable").astype(np.int32)
 
-    def tone_order(self) -> np.ndarray:
+    def tone_order(self, high_first: bool = True) -> np.ndarray:
         """Longest block first, then the blocks of higher-frequency tones first
-        (ties in table order).  With a guess (extrapolated or learned) a tone's
-        Jacobi sweeps per step follow its frequency closely (more sweeps at
-        higher frequency), so lockstep groups of neighbouring frequencies wait
-        less for their slowest block at each round, and the heaviest groups are
-        dealt out first.  For cm_set_bm_order; results do not depend on it."""
+        (low-frequency first with high_first=False; ties in table order).  A
+        tone's Jacobi sweeps per step follow its frequency closely: with a
+        guess (extrapolated or learned) they rise with frequency, without one
+        (plain) they fall.  Taking the heavy tones first deals the heaviest
+        work out first, and lockstep groups of neighbouring frequencies wait
+        less for their slowest block at each round.  For cm_set_bm_order;
+        results do not depend on it."""
         lens = self.items["mesi"].astype(np.int64) - self.items["start"]
         tone = np.arange(self.TB) // self.DF
-        return np.lexsort((np.arange(self.TB), -self.freqs[tone], -lens)).astype(np.int32)
+        key = -self.freqs[tone] if high_first else self.freqs[tone]
+        return np.lexsort((np.arange(self.TB), key, -lens)).astype(np.int32)
 
     def stats(self):
         s = self.d_stats.view(self.TB, 3).cpu().numpy().astype(np.int64)

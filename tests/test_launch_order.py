@@ -63,3 +63,21 @@ def test_tone_order():
     long_part = o[lens[o] == lens.max()]
     for g in range(0, len(long_part), 16):
         assert len(np.unique(tone[long_part[g:g + 16]])) <= 3
+
+
+def test_tone_order_low_first():
+    tb = _stub(29, 3)
+    o = batch.ToneBatch.tone_order(tb, high_first=False)
+    hi = batch.ToneBatch.tone_order(tb)
+    lens = tb.items["mesi"].astype(np.int64) - tb.items["start"]
+    tone = np.arange(tb.TB) // tb.DF
+    assert sorted(o.tolist()) == list(range(tb.TB))
+    assert np.all(np.diff(lens[o]) <= 0)
+    for L in np.unique(lens):
+        sel = o[lens[o] == L]
+        f = tb.freqs[tone[sel]]
+        assert np.all(np.diff(f) >= 0)            # lower frequency first
+        # the same tones as high-first, in reverse, each tone's blocks in table order
+        rev = hi[lens[hi] == L]
+        assert np.array_equal(np.unique(tone[sel]), np.unique(tone[rev]))
+        assert np.all(np.diff(sel)[np.diff(f) == 0] > 0)
